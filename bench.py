@@ -1,0 +1,206 @@
+"""Throughput benchmark of the hip multislice path on one MI355X.
+
+Workload (the headline metric's configuration): 512 x 512 waves, 1 A pixel,
+100 slices of 1 A, 200 keV, Cs 1.3 mm, defocus 700 A, aperture 10 mrad,
+synthetic protein-like clouds (H/C/N/O/S, 1 A minimum spacing) of
+log-uniform 10k-100k atoms in random orientations, no water.  One step is
+one pass of the full path over one batch of molecules whose atoms are
+already in GPU memory: binning -> potential slices -> multislice -> lens ->
+|psi|^2 (emsim_amd.batch.BatchPipe).
+
+Prints one JSON line.  ``roofline`` describes the kernel with the largest
+share of GPU time in the timed region: its algorithmic bytes per launch
+(fixed per-pixel figures, see KERNEL_BYTES) divided by its mean launch
+duration, both measured live with HIP events on the launch stream.
+``cpu_baseline`` times the CPU oracle (numpy, float32 fftw semantics) on
+one image of the same workload.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--slices", type=int, default=100)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-timing", action="store_true",
+                    help="do not bracket launches with HIP events")
+    return ap.parse_args()
+
+
+def workload(batch: int, n_atoms_range=(10000, 100000), n_base: int = 8, seed: int = 10000):
+    """Batch of molecules: n_base base clouds, each image a random rotation of one."""
+    from emsim_amd import synth
+    from emsim_amd.utils.rot import random_uniform_quaternions
+    rng = np.random.default_rng(seed)
+    lo, hi = np.log(n_atoms_range[0]), np.log(n_atoms_range[1])
+    sizes = np.exp(rng.uniform(lo, hi, n_base)).astype(int)
+    bases = [synth.protein_cloud(int(s), seed + 1 + i) for i, s in enumerate(sizes)]
+    quats = random_uniform_quaternions(batch, rng=np.random.default_rng(2))
+    return [synth.rotated(bases[i % n_base], quats[i]) for i in range(batch)], sizes
+
+
+def kernel_bytes_per_launch(name: str, n: int, cfg) -> float:
+    """Algorithmic HBM bytes of one launch of a kernel in this workload.
+
+    Per real pixel per slice (h = (n/2+1)/n, nE elements):
+      k_row_transmit_fft   pass A: wave in 8 + V 4 + wave out 8      = 20
+      k_col_fft_mul_ifft   pass B: wave in 8 + out 8                 = 16
+      k_row_r2c_pairs      hist in 4 nE + half spectra out 8h nE
+      k_col_spectrum       half spectra in 8h nE + out 8h (+ table 4h nE once)
+      k_row_c2r_pairs      half spectrum in 8h + V out 4
+      k_bin_tiles          histogram out 4 nE
+      k_row_fft            wave in 8 + out 8
+      k_row_ifft_intensity wave in 8 + image out 4
+    One pass-A/B launch covers the batch; one potential-build launch covers
+    the slices of one molecule.
+    """
+    B, npix, nS, nE = cfg["batch"], cfg["n"] ** 2, cfg["slices"], cfg["n_elems"]
+    h = (cfg["n"] // 2 + 1) / cfg["n"]
+    per = {
+        "k_row_transmit_fft": 20.0 * npix * B,
+        "k_col_fft_mul_ifft": 16.0 * npix * B,
+        "k_row_fft": 16.0 * npix * B,
+        "k_row_ifft_intensity": 12.0 * npix * B,
+        "k_row_r2c_pairs": (4.0 * nE + 8 * h * nE) * npix * nS,
+        "k_col_spectrum": (8 * h * nE + 8 * h) * npix * nS + 4 * h * nE * npix,
+        "k_row_c2r_pairs": (8 * h + 4.0) * npix * nS,
+        "k_bin_tiles": 4.0 * nE * npix * nS,
+    }
+    for k, v in per.items():
+        if name.startswith(k):
+            return v
+    return float("nan")
+
+
+def cpu_baseline(mols, cfg, tem):
+    """CPU oracle (numpy float32 fftw semantics, pocketfft single thread) on one image."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import emsim_oracle as orc
+    from emsim_amd import atoms as atm
+    n, nS, dz, d = cfg["n"], cfg["slices"], cfg["dz"], cfg["pixel"]
+    lam, gamma = orc.wavelength_and_gamma(tem.beam_energy_kev)
+    mol = atm.centralize(mols[0])
+    mol, zs, counts = atm.sort_elements_and_count(mol, cfg["elements"])
+    sf = orc.scattering_factors_half(list(zs), n, n, d, d)
+    t0 = time.perf_counter()
+    hist = orc.bin_atoms_multi_slices(mol.coordinates, counts, nS, n, n, dz, d, d)
+    v = orc.make_multi_slices_fftw32(hist, sf).astype(np.float32)
+    w = orc.init_wave(n, n, d, d, tem.electron_dose)
+    w = orc.multislice_fftw32(w, v, dz, lam, gamma, d, d)
+    img = orc.intensity(orc.lens_propagate_fftw32(w, lam, tem.cs_mm, tem.defocus,
+                                                  tem.aperture, d, d))
+    dt = time.perf_counter() - t0
+    assert np.isfinite(img).all()
+    return {"value": 1.0 / dt, "unit": "images/s", "cores": 1, "kind": "port",
+            "sample": f"1 image of the workload ({mols[0].label or len(mols[0].elements)} atoms, "
+                      f"{n}x{n}, all {nS} slices), oracle/emsim_oracle.py numpy float32 "
+                      f"restatement of the fftw backend, {dt:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(images shard across GPUs with no collective; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import emsim_amd
+    from emsim_amd.backend import hip_ext
+    from emsim_amd.batch import BatchPipe
+
+    cfg = dict(n=args.n, slices=args.slices, pixel=1.0, dz=1.0, batch=args.batch,
+               elements=[1, 6, 7, 8, 16], n_elems=5)
+    tem = emsim_amd.tem.TEM(electron_dose=20, beam_energy_kev=200, cs_mm=1.3, defocus=700,
+                            aperture=10e-3)
+    mols, sizes = workload(args.batch)
+    pipe = BatchPipe(tem, cfg["pixel"], cfg["dz"], cfg["n"], cfg["slices"], cfg["elements"])
+    prepared = [pipe.prepare(m) for m in mols]
+    torch.cuda.synchronize()
+
+    for _ in range(args.warmup):
+        pipe.run(prepared)
+    torch.cuda.synchronize()
+
+    if not args.no_timing:
+        hip_ext.timing_collect()
+        hip_ext.timing_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        images = pipe.run(prepared)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    kernels = []
+    if not args.no_timing:
+        hip_ext.timing_enable(False)
+        kernels = hip_ext.timing_collect()
+
+    assert torch.isfinite(images).all().item()
+    n_images = args.batch * args.steps
+    value = n_images / elapsed
+
+    roofline = None
+    ktable = []
+    gpu_total = sum(k["total_ms"] for k in kernels)
+    for k in sorted(kernels, key=lambda r: -r["total_ms"]):
+        b = kernel_bytes_per_launch(k["name"], k["n"], cfg)
+        gbs = b / (k["mean_ms"] * 1e-3) / 1e9 if math.isfinite(b) else None
+        ktable.append({"kernel": f"{k['name']}", "launches": k["count"],
+                       "mean_us": round(k["mean_ms"] * 1e3, 2),
+                       "share": round(k["total_ms"] / gpu_total, 4),
+                       "alg_GBs": None if gbs is None else round(gbs, 1)})
+    if ktable:
+        top = next((r for r in ktable if r["alg_GBs"] is not None), None)
+        if top is not None:
+            roofline = {"bound": "hbm", "achieved": top["alg_GBs"], "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(top["alg_GBs"] / HBM_PEAK_GBS, 4),
+                        "traffic": None, "kernel": top["kernel"], "mean_us": top["mean_us"]}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(mols, cfg, tem)
+
+    line = {
+        "metric": "images/s at 512^2 x 100 slices (plain multislice, full path atoms->image)",
+        "value": round(value, 2),
+        "unit": "images/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32/c64",
+        "data": "synthetic",
+        "config": {"workload": f"plain multislice {cfg['n']}^2 x {cfg['slices']} slices, "
+                               f"dz 1 A, pixel 1 A, batch {args.batch} molecules of "
+                               f"{int(sizes.min())}-{int(sizes.max())} atoms (H/C/N/O/S), "
+                               f"random orientations, 200 keV, Cs 1.3 mm, df 700 A, 10 mrad",
+                   "batch": args.batch, "n": cfg["n"], "slices": cfg["slices"]},
+        "slices_per_s": round(value * cfg["slices"], 1),
+        "gpu_busy_frac": round(gpu_total * 1e-3 / elapsed, 4) if kernels else None,
+        "roofline": roofline,
+        "kernels": ktable,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

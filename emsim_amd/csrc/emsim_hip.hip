@@ -1,0 +1,884 @@
+// C ABI of the gfx950 multislice backend (declarations: include/emsim_hip.h).
+//
+// Host side only: argument checks, float32 constants computed in the
+// reference's operation order, constant-table uploads, and launches of the
+// kernels in kernels_wave.hpp / kernels_slice.hpp on the caller's stream.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/emsim_hip.h"
+#include "fft_lds.hpp"
+#include "kernels_slice.hpp"
+#include "kernels_wave.hpp"
+
+using namespace emsim;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define EMSIM_HIP_CHECK(call)                                                          \
+    do {                                                                                \
+        hipError_t err_ = (call);                                                       \
+        if (err_ != hipSuccess)                                                         \
+            return fail(EMSIM_HIP_EDEVICE, "%s failed: %s", #call, hipGetErrorString(err_)); \
+    } while (0)
+
+// --------------------------------------------------------------------------
+// launches: error check plus optional per-kernel event timing
+// (emsim_hip_timing_enable / emsim_hip_timing_collect).  Events are recorded
+// on the launch stream, immediately before and after the kernel.
+// --------------------------------------------------------------------------
+struct TimingRecord {
+    const char* name;
+    int n;
+    hipEvent_t start, stop;
+};
+std::mutex g_timing_mutex;
+bool g_timing_on = false;
+std::vector<TimingRecord> g_timing_records;
+std::vector<hipEvent_t> g_event_pool;
+
+hipEvent_t pooled_event() {
+    if (!g_event_pool.empty()) {
+        hipEvent_t e = g_event_pool.back();
+        g_event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+hipEvent_t timing_begin(hipStream_t s) {
+    std::lock_guard<std::mutex> lock(g_timing_mutex);
+    if (!g_timing_on) return nullptr;
+    hipEvent_t e = pooled_event();
+    if (e && hipEventRecord(e, s) != hipSuccess) {
+        g_event_pool.push_back(e);
+        return nullptr;
+    }
+    return e;
+}
+
+void timing_end(hipEvent_t start, const char* name, int n, hipStream_t s) {
+    if (!start) return;
+    std::lock_guard<std::mutex> lock(g_timing_mutex);
+    hipEvent_t e = pooled_event();
+    if (!e || hipEventRecord(e, s) != hipSuccess) {
+        g_event_pool.push_back(start);
+        if (e) g_event_pool.push_back(e);
+        return;
+    }
+    g_timing_records.push_back({name, n, start, e});
+}
+
+#define EMSIM_LAUNCH(tag_n, kernel, grid, block, shmem, stream, ...)                      \
+    do {                                                                                \
+        hipEvent_t tk_ = timing_begin(stream);                                          \
+        hipLaunchKernelGGL(kernel, grid, block, shmem, stream, __VA_ARGS__);            \
+        hipError_t err_ = hipGetLastError();                                            \
+        if (err_ != hipSuccess)                                                         \
+            return fail(EMSIM_HIP_EDEVICE, "launch of %s failed: %s", #kernel,          \
+                        hipGetErrorString(err_));                                       \
+        timing_end(tk_, #kernel, tag_n, stream);                                        \
+    } while (0)
+
+// The reference's PI literal (WavePropagator.cpp:10) as float32.
+constexpr float kPi32 = 3.14159265358979324f;
+
+bool pow2_supported(int n) { return n >= 8 && n <= kTwN && (n & (n - 1)) == 0; }
+
+// --------------------------------------------------------------------------
+// per-device twiddle table w[m] = exp(-2 pi i m / 4096), float64 -> float32
+// --------------------------------------------------------------------------
+std::mutex g_tw_mutex;
+std::vector<float2*> g_twiddles;
+
+int get_twiddles(const float2** out) {
+    int dev = 0;
+    EMSIM_HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_tw_mutex);
+    if ((int)g_twiddles.size() <= dev) g_twiddles.resize(dev + 1, nullptr);
+    if (!g_twiddles[dev]) {
+        std::vector<float2> host(kTwN);
+        const double two_pi = 6.283185307179586476925286766559;
+        for (int m = 0; m < kTwN; ++m) {
+            const double a = -two_pi * (double)m / (double)kTwN;
+            host[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+        }
+        float2* d = nullptr;
+        EMSIM_HIP_CHECK(hipMalloc(&d, kTwN * sizeof(float2)));
+        hipError_t err = hipMemcpy(d, host.data(), kTwN * sizeof(float2), hipMemcpyHostToDevice);
+        if (err != hipSuccess) {
+            (void)hipFree(d);
+            return fail(EMSIM_HIP_EDEVICE, "twiddle upload failed: %s", hipGetErrorString(err));
+        }
+        g_twiddles[dev] = d;
+    }
+    *out = g_twiddles[dev];
+    return EMSIM_HIP_OK;
+}
+
+// --------------------------------------------------------------------------
+// numpy.arange(start, stop, step) for float64, as the reference's binning
+// calls it (slice_builder_base.py:183-186): length ceil((stop-start)/step),
+// element 0 = start, element 1 = start + step, element i >= 2 =
+// start + i * ((start + step) - start).
+// --------------------------------------------------------------------------
+void numpy_arange(double start, double stop, double step, std::vector<double>& out) {
+    out.clear();
+    const double len_f = std::ceil((stop - start) / step);
+    if (!(len_f > 0)) return;
+    const long len = (long)len_f;
+    out.resize(len);
+    out[0] = start;
+    if (len == 1) return;
+    const double next = start + step;
+    out[1] = next;
+    const double delta = next - start;
+    for (long i = 2; i < len; ++i) out[i] = start + (double)i * delta;
+}
+
+void reference_edges(int n, double d, std::vector<double>& e64) {
+    const double start = -d * (double)(n / 2);
+    const double end = start + (double)n * d;
+    numpy_arange(start, end + d, d, e64);
+}
+
+// Smallest float32 >= x.
+float round_up_f32(double x) {
+    float f = (float)x;
+    if ((double)f < x) f = std::nextafter(f, INFINITY);
+    return f;
+}
+
+int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+int grid_for(int64_t n, int block) {
+    int64_t g = ceil_div(n, block);
+    if (g < 1) g = 1;
+    if (g > 65536) g = 65536;
+    return (int)g;
+}
+
+}  // namespace
+
+// ==========================================================================
+// handles
+// ==========================================================================
+struct emsim_hip_slice_builder {
+    int n_elems = 0, n_slices = 0, n1 = 0, n2 = 0, H = 0;
+    double dz = 0, d1 = 0, d2 = 0;
+    int one_slice = 0;
+    float* sf = nullptr;        // (n_elems, n1, H)
+    float* edges = nullptr;     // axis0 (n0+1) | axis1 (n1+1) | axis2 (n2+1)
+    int last_exact[3] = {0, 0, 0};
+    float inv_d[3] = {0, 0, 0};
+    int tile_rows = 1, tiles_per_plane = 1;
+    const float2* tw = nullptr;
+};
+
+struct emsim_hip_wave_propagator {
+    int n1 = 0, n2 = 0;
+    float d1 = 0, d2 = 0, lam = 0, gamma = 0;
+    float dfx = 0, dfy = 0, filter2 = 0, factor = 0;
+    const float2* tw = nullptr;
+};
+
+// ==========================================================================
+// size dispatch
+// ==========================================================================
+#define EMSIM_DISPATCH(n, FN, ...)                                                  \
+    switch (n) {                                                                    \
+        case 8: return FN<8>(__VA_ARGS__);                                          \
+        case 16: return FN<16>(__VA_ARGS__);                                        \
+        case 32: return FN<32>(__VA_ARGS__);                                        \
+        case 64: return FN<64>(__VA_ARGS__);                                        \
+        case 128: return FN<128>(__VA_ARGS__);                                      \
+        case 256: return FN<256>(__VA_ARGS__);                                      \
+        case 512: return FN<512>(__VA_ARGS__);                                      \
+        case 1024: return FN<1024>(__VA_ARGS__);                                    \
+        case 2048: return FN<2048>(__VA_ARGS__);                                    \
+        case 4096: return FN<4096>(__VA_ARGS__);                                    \
+        default: return fail(EMSIM_HIP_EUNSUPPORTED, "unsupported FFT length %d", n); \
+    }
+
+namespace {
+
+template <int N>
+int launch_row_fft(bool inv, const float2* in, float2* out, int64_t rows, const float2* tw,
+                   hipStream_t s) {
+    using Cfg = RowCfg<N>;
+    const dim3 grid((unsigned)ceil_div(rows, Cfg::ROWS));
+    if (inv) EMSIM_LAUNCH(N, (k_row_fft<N, true>), grid, dim3(Cfg::BLOCK), 0, s, in, out, rows, tw);
+    else EMSIM_LAUNCH(N, (k_row_fft<N, false>), grid, dim3(Cfg::BLOCK), 0, s, in, out, rows, tw);
+    return EMSIM_HIP_OK;
+}
+
+template <int N>
+int launch_row_transmit_fft(bool first, const float2* in, const float* V, float2* out,
+                            int64_t rows, int rows_per_img, int64_t v_img_stride, float factor,
+                            const float2* tw, hipStream_t s) {
+    using Cfg = RowCfg<N>;
+    const dim3 grid((unsigned)ceil_div(rows, Cfg::ROWS));
+    if (first)
+        EMSIM_LAUNCH(N, (k_row_transmit_fft<N, true>), grid, dim3(Cfg::BLOCK), 0, s, in, V,
+                           out, rows, rows_per_img, v_img_stride, factor, tw);
+    else
+        EMSIM_LAUNCH(N, (k_row_transmit_fft<N, false>), grid, dim3(Cfg::BLOCK), 0, s, in, V,
+                           out, rows, rows_per_img, v_img_stride, factor, tw);
+    return EMSIM_HIP_OK;
+}
+
+template <int N>
+int launch_row_ifft_intensity(const float2* in, float* out, int64_t rows, const float2* tw,
+                              hipStream_t s) {
+    using Cfg = RowCfg<N>;
+    EMSIM_LAUNCH(N, (k_row_ifft_intensity<N>), dim3((unsigned)ceil_div(rows, Cfg::ROWS)),
+                       dim3(Cfg::BLOCK), 0, s, in, out, rows, tw);
+    return EMSIM_HIP_OK;
+}
+
+template <int N, class Mul>
+int launch_col_mul_impl(float2* data, int ld, int ncols, int64_t img_stride, int n_img, Mul mul,
+                        const float2* tw, hipStream_t s) {
+    using Cfg = ColCfg<N>;
+    const dim3 grid((unsigned)ceil_div(ncols, Cfg::C), (unsigned)n_img);
+    EMSIM_LAUNCH(N, (k_col_fft_mul_ifft<N, Mul>), grid, dim3(Cfg::BLOCK), 0, s, data, ld,
+                       ncols, img_stride, mul, tw);
+    return EMSIM_HIP_OK;
+}
+
+template <int N>
+int launch_col_prop(float2* data, int ld, int ncols, int64_t img_stride, int n_img,
+                    PropagatorMul mul, const float2* tw, hipStream_t s) {
+    return launch_col_mul_impl<N>(data, ld, ncols, img_stride, n_img, mul, tw, s);
+}
+
+template <int N>
+int launch_col_lens(float2* data, int ld, int ncols, int64_t img_stride, int n_img, LensMul mul,
+                    const float2* tw, hipStream_t s) {
+    return launch_col_mul_impl<N>(data, ld, ncols, img_stride, n_img, mul, tw, s);
+}
+
+template <int N>
+int launch_r2c_pairs(const float* in, float2* out, int64_t n_pairs, int rows_per_plane,
+                     int planes_per_group, int64_t in_group_stride, int64_t in_plane0,
+                     const float2* tw, hipStream_t s) {
+    using Cfg = RowCfg<N>;
+    EMSIM_LAUNCH(N, (k_row_r2c_pairs<N>), dim3((unsigned)ceil_div(n_pairs, Cfg::ROWS)),
+                       dim3(Cfg::BLOCK), 0, s, in, out, n_pairs, rows_per_plane, planes_per_group,
+                       in_group_stride, in_plane0, tw);
+    return EMSIM_HIP_OK;
+}
+
+template <int N>
+int launch_c2r_pairs(const float2* in, float* out, int64_t n_pairs, float clip_lo,
+                     const float2* tw, hipStream_t s) {
+    using Cfg = RowCfg<N>;
+    EMSIM_LAUNCH(N, (k_row_c2r_pairs<N>), dim3((unsigned)ceil_div(n_pairs, Cfg::ROWS)),
+                       dim3(Cfg::BLOCK), 0, s, in, out, n_pairs, clip_lo, tw);
+    return EMSIM_HIP_OK;
+}
+
+template <int N>
+int launch_col_spectrum(float2* X, const float* sf, int n_elems, int n_sc, int H, int n2,
+                        int bandlimit, int band_r2, float inv_npix, const float2* tw,
+                        hipStream_t s) {
+    using Cfg = ColCfg<N>;
+    const dim3 grid((unsigned)ceil_div(H, Cfg::C), (unsigned)n_sc);
+    EMSIM_LAUNCH(N, (k_col_spectrum<N>), grid, dim3(Cfg::BLOCK), 0, s, X, sf, n_elems, n_sc, H,
+                       n2, bandlimit, band_r2, inv_npix, tw);
+    return EMSIM_HIP_OK;
+}
+
+int row_fft(int n, bool inv, const float2* in, float2* out, int64_t rows, const float2* tw,
+            hipStream_t s) {
+    EMSIM_DISPATCH(n, launch_row_fft, inv, in, out, rows, tw, s);
+}
+int row_transmit_fft(int n, bool first, const float2* in, const float* V, float2* out,
+                     int64_t rows, int rows_per_img, int64_t v_img_stride, float factor,
+                     const float2* tw, hipStream_t s) {
+    EMSIM_DISPATCH(n, launch_row_transmit_fft, first, in, V, out, rows, rows_per_img,
+                   v_img_stride, factor, tw, s);
+}
+int row_ifft_intensity(int n, const float2* in, float* out, int64_t rows, const float2* tw,
+                       hipStream_t s) {
+    EMSIM_DISPATCH(n, launch_row_ifft_intensity, in, out, rows, tw, s);
+}
+int col_prop(int n, float2* data, int ld, int ncols, int64_t img_stride, int n_img,
+             PropagatorMul mul, const float2* tw, hipStream_t s) {
+    EMSIM_DISPATCH(n, launch_col_prop, data, ld, ncols, img_stride, n_img, mul, tw, s);
+}
+int col_lens(int n, float2* data, int ld, int ncols, int64_t img_stride, int n_img, LensMul mul,
+             const float2* tw, hipStream_t s) {
+    EMSIM_DISPATCH(n, launch_col_lens, data, ld, ncols, img_stride, n_img, mul, tw, s);
+}
+int r2c_pairs(int n, const float* in, float2* out, int64_t n_pairs, int rows_per_plane,
+              int planes_per_group, int64_t in_group_stride, int64_t in_plane0, const float2* tw,
+              hipStream_t s) {
+    EMSIM_DISPATCH(n, launch_r2c_pairs, in, out, n_pairs, rows_per_plane, planes_per_group,
+                   in_group_stride, in_plane0, tw, s);
+}
+int c2r_pairs(int n, const float2* in, float* out, int64_t n_pairs, float clip_lo,
+              const float2* tw, hipStream_t s) {
+    EMSIM_DISPATCH(n, launch_c2r_pairs, in, out, n_pairs, clip_lo, tw, s);
+}
+int col_spectrum(int n, float2* X, const float* sf, int n_elems, int n_sc, int H, int n2,
+                 int bandlimit, int band_r2, float inv_npix, const float2* tw, hipStream_t s) {
+    EMSIM_DISPATCH(n, launch_col_spectrum, X, sf, n_elems, n_sc, H, n2, bandlimit, band_r2,
+                   inv_npix, tw, s);
+}
+
+// Float32 constants of the propagators, in the reference's operation order
+// (WavePropagator.cpp:19-23,62-85,108-130).
+PropagatorMul make_propagator(const emsim_hip_wave_propagator* w, float dz) {
+    PropagatorMul p;
+    p.n1 = w->n1;
+    p.n2 = w->n2;
+    p.dfx = w->dfx;
+    p.dfy = w->dfy;
+    p.filter2 = w->filter2;
+    const float lp = w->lam * kPi32;
+    p.lpd = lp * dz;
+    p.inv_npix = 1.0f / (float)(w->n1 * w->n2);
+    return p;
+}
+
+LensMul make_lens(const emsim_hip_wave_propagator* w, float cs_mm, float defocus, float aperture) {
+    LensMul l;
+    l.n1 = w->n1;
+    l.n2 = w->n2;
+    l.dfx = w->dfx;
+    l.dfy = w->dfy;
+    float c1 = 0.5f * kPi32;
+    c1 = c1 * cs_mm;
+    c1 = c1 * 1e7f;
+    c1 = c1 * w->lam;
+    c1 = c1 * w->lam;
+    c1 = c1 * w->lam;
+    float c2 = kPi32 * defocus;
+    c2 = c2 * w->lam;
+    const float f_aper = aperture / w->lam;
+    l.c1 = c1;
+    l.c2 = c2;
+    l.faper2 = f_aper * f_aper;
+    l.inv_npix = 1.0f / (float)(w->n1 * w->n2);
+    return l;
+}
+
+}  // namespace
+
+// ==========================================================================
+// library
+// ==========================================================================
+extern "C" int emsim_hip_abi_version(void) { return EMSIM_HIP_ABI_VERSION; }
+
+extern "C" const char* emsim_hip_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int emsim_hip_fft_size_supported(int n) { return pow2_supported(n) ? 1 : 0; }
+
+extern "C" int emsim_hip_bin_edges(int n, double d, double* edges64_host, float* edges32_host,
+                                   int cap, int* len_out) {
+    if (n < 1 || !(d > 0) || !len_out) return fail(EMSIM_HIP_EINVAL, "bin_edges: bad arguments");
+    std::vector<double> e;
+    reference_edges(n, d, e);
+    *len_out = (int)e.size();
+    for (int i = 0; i < (int)e.size() && i < cap; ++i) {
+        if (edges64_host) edges64_host[i] = e[i];
+        if (edges32_host) edges32_host[i] = round_up_f32(e[i]);
+    }
+    return EMSIM_HIP_OK;
+}
+
+// ==========================================================================
+// slice builder
+// ==========================================================================
+extern "C" int emsim_hip_slice_builder_create(emsim_hip_slice_builder** out,
+                                              const float* sf_host, int n_elems, int n_slices,
+                                              int n1, int n2, double dz, double d1, double d2,
+                                              int one_slice) {
+    if (!out) return fail(EMSIM_HIP_EINVAL, "slice_builder_create: null output handle");
+    *out = nullptr;
+    if (!sf_host || n_elems < 1 || n_slices < 1)
+        return fail(EMSIM_HIP_EINVAL, "slice_builder_create: need n_elems >= 1, n_slices >= 1 "
+                                      "and a scattering-factor table");
+    if (n_elems > kMaxElems)
+        return fail(EMSIM_HIP_EUNSUPPORTED, "slice_builder_create: at most %d elements", kMaxElems);
+    if (!pow2_supported(n1) || !pow2_supported(n2))
+        return fail(EMSIM_HIP_EUNSUPPORTED,
+                    "slice_builder_create: lateral size %dx%d; the hip backend implements "
+                    "powers of two from 8 to %d", n1, n2, kTwN);
+    if (one_slice && n_slices != 1)
+        return fail(EMSIM_HIP_EINVAL, "slice_builder_create: one-slice builder needs n_slices == 1");
+    if (!(d1 > 0) || !(d2 > 0) || (!one_slice && !(dz > 0)))
+        return fail(EMSIM_HIP_EINVAL, "slice_builder_create: voxel sizes must be positive");
+
+    // Edge tables (float64 numpy.arange semantics, then rounded up to float32).
+    const int ns[3] = {one_slice ? 1 : n_slices, n1, n2};
+    const double ds[3] = {one_slice ? 1.0 : dz, d1, d2};
+    std::vector<float> edges32;
+    int last_exact[3];
+    float inv_d[3];
+    for (int a = 0; a < 3; ++a) {
+        std::vector<double> e;
+        reference_edges(ns[a], ds[a], e);
+        if ((int)e.size() != ns[a] + 1) {
+            if (a == 0 && one_slice) {
+                e.resize(2);  // axis 0 is ignored by the one-slice builder
+                e[0] = -1.0;
+                e[1] = 1.0;
+            } else {
+                return fail(EMSIM_HIP_EUNSUPPORTED,
+                            "binning axis %d: numpy.arange makes %d edges for %d bins of %.17g "
+                            "(the reference would return a %d-bin histogram on this axis and "
+                            "misalign its slices); choose a pixel size that divides evenly",
+                            a, (int)e.size(), ns[a], ds[a], (int)e.size() - 1);
+            }
+        }
+        for (double x : e) edges32.push_back(round_up_f32(x));
+        last_exact[a] = ((double)(float)e.back() == e.back()) ? 1 : 0;
+        inv_d[a] = (float)(1.0 / ds[a]);
+    }
+
+    emsim_hip_slice_builder* b = new (std::nothrow) emsim_hip_slice_builder();
+    if (!b) return fail(EMSIM_HIP_ENOMEM, "slice_builder_create: out of host memory");
+    b->n_elems = n_elems;
+    b->n_slices = n_slices;
+    b->n1 = n1;
+    b->n2 = n2;
+    b->H = n2 / 2 + 1;
+    b->dz = dz;
+    b->d1 = d1;
+    b->d2 = d2;
+    b->one_slice = one_slice ? 1 : 0;
+    for (int a = 0; a < 3; ++a) {
+        b->last_exact[a] = last_exact[a];
+        b->inv_d[a] = inv_d[a];
+    }
+    int rows = kBinTileWords / n2;
+    if (rows < 1) rows = 1;
+    if (rows > n1) rows = n1;
+    b->tile_rows = rows;
+    b->tiles_per_plane = (int)ceil_div(n1, rows);
+
+    int rc = get_twiddles(&b->tw);
+    if (rc) {
+        delete b;
+        return rc;
+    }
+    const size_t sf_bytes = (size_t)n_elems * n1 * b->H * sizeof(float);
+    hipError_t err = hipMalloc(&b->sf, sf_bytes);
+    if (err == hipSuccess) err = hipMemcpy(b->sf, sf_host, sf_bytes, hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipMalloc(&b->edges, edges32.size() * sizeof(float));
+    if (err == hipSuccess)
+        err = hipMemcpy(b->edges, edges32.data(), edges32.size() * sizeof(float),
+                        hipMemcpyHostToDevice);
+    if (err != hipSuccess) {
+        emsim_hip_slice_builder_destroy(b);
+        return fail(EMSIM_HIP_EDEVICE, "slice_builder_create: %s", hipGetErrorString(err));
+    }
+    *out = b;
+    return EMSIM_HIP_OK;
+}
+
+extern "C" int emsim_hip_slice_builder_destroy(emsim_hip_slice_builder* b) {
+    if (!b) return EMSIM_HIP_OK;
+    if (b->sf) (void)hipFree(b->sf);
+    if (b->edges) (void)hipFree(b->edges);
+    delete b;
+    return EMSIM_HIP_OK;
+}
+
+namespace {
+struct BinWorkspace {
+    int32_t* atom_tile;
+    uint32_t* atom_local;
+    uint32_t* bucket;
+    uint32_t* tile_count;
+    uint32_t* tile_start;
+    uint32_t* block_sums;
+    size_t bytes;
+};
+
+size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+BinWorkspace bin_layout(const emsim_hip_slice_builder* b, int64_t n_atoms, char* base) {
+    const int64_t n0 = b->one_slice ? 1 : b->n_slices;
+    const int64_t n_tiles = (int64_t)b->n_elems * n0 * b->tiles_per_plane;
+    const int64_t n_scan = n_tiles + 1;
+    const int64_t n_blocks = ceil_div(n_scan, kScanTile);
+    BinWorkspace w;
+    size_t off = 0;
+    w.atom_tile = (int32_t*)(base + off);  off += align_up(sizeof(int32_t) * (size_t)n_atoms);
+    w.atom_local = (uint32_t*)(base + off); off += align_up(sizeof(uint32_t) * (size_t)n_atoms);
+    w.bucket = (uint32_t*)(base + off);     off += align_up(sizeof(uint32_t) * (size_t)n_atoms);
+    w.tile_count = (uint32_t*)(base + off); off += align_up(sizeof(uint32_t) * (size_t)n_scan);
+    w.tile_start = (uint32_t*)(base + off); off += align_up(sizeof(uint32_t) * (size_t)n_scan);
+    w.block_sums = (uint32_t*)(base + off); off += align_up(sizeof(uint32_t) * (size_t)n_blocks);
+    w.bytes = off;
+    return w;
+}
+}  // namespace
+
+extern "C" size_t emsim_hip_bin_workspace_size(const emsim_hip_slice_builder* b, int64_t n_atoms) {
+    if (!b || n_atoms < 0) return 0;
+    return bin_layout(b, n_atoms, nullptr).bytes;
+}
+
+extern "C" int emsim_hip_bin_atoms(const emsim_hip_slice_builder* b, const float* coords,
+                                   const int64_t* counts_host, int64_t n_atoms, float* hist,
+                                   void* workspace, size_t workspace_bytes,
+                                   emsim_stream_t stream_) {
+    hipStream_t s = (hipStream_t)stream_;
+    if (!b || !hist || !counts_host || n_atoms < 0 || (n_atoms > 0 && !coords))
+        return fail(EMSIM_HIP_EINVAL, "bin_atoms: bad arguments");
+    if (n_atoms >= (int64_t)0x7fffffff)
+        return fail(EMSIM_HIP_EUNSUPPORTED, "bin_atoms: at most 2^31-1 atoms per call");
+    ElemOffsets eo;
+    eo.n_elems = b->n_elems;
+    eo.off[0] = 0;
+    for (int e = 0; e < b->n_elems; ++e) {
+        if (counts_host[e] < 0) return fail(EMSIM_HIP_EINVAL, "bin_atoms: negative count");
+        eo.off[e + 1] = eo.off[e] + counts_host[e];
+    }
+    if (eo.off[b->n_elems] > n_atoms)
+        return fail(EMSIM_HIP_EINVAL, "bin_atoms: counts sum to %lld but only %lld atoms given",
+                    (long long)eo.off[b->n_elems], (long long)n_atoms);
+    BinWorkspace w = bin_layout(b, n_atoms, (char*)workspace);
+    if (!workspace || workspace_bytes < w.bytes)
+        return fail(EMSIM_HIP_EWORKSPACE, "bin_atoms: workspace needs %zu bytes", w.bytes);
+
+    const int64_t n0 = b->one_slice ? 1 : b->n_slices;
+    BinGeom g;
+    g.e0 = b->edges;
+    g.e1 = b->edges + (n0 + 1);
+    g.e2 = g.e1 + (b->n1 + 1);
+    g.n0 = (int)n0;
+    g.n1 = b->n1;
+    g.n2 = b->n2;
+    g.last0 = b->last_exact[0];
+    g.last1 = b->last_exact[1];
+    g.last2 = b->last_exact[2];
+    g.inv0 = b->inv_d[0];
+    g.inv1 = b->inv_d[1];
+    g.inv2 = b->inv_d[2];
+    g.one_slice = b->one_slice;
+    g.tile_rows = b->tile_rows;
+    g.tiles_per_plane = b->tiles_per_plane;
+
+    const int64_t n_tiles = (int64_t)b->n_elems * n0 * b->tiles_per_plane;
+    const int64_t n_scan = n_tiles + 1;
+    const int64_t n_blocks = ceil_div(n_scan, kScanTile);
+
+    EMSIM_LAUNCH(0, k_fill_u32, dim3(grid_for(n_scan, 256)), dim3(256), 0, s, w.tile_count,
+                       n_scan, 0u);
+    if (n_atoms > 0) {
+        EMSIM_LAUNCH(0, k_bin_classify, dim3(grid_for(n_atoms, 256)), dim3(256), 0, s, coords,
+                           n_atoms, eo, g, w.atom_tile, w.atom_local, w.tile_count);
+    }
+    EMSIM_LAUNCH(0, k_scan_tiles, dim3((unsigned)n_blocks), dim3(kScanBlock), 0, s,
+                       w.tile_count, w.tile_start, n_scan, w.block_sums);
+    EMSIM_LAUNCH(0, k_scan_sums, dim3(1), dim3(kScanBlock), 0, s, w.block_sums, n_blocks);
+    EMSIM_LAUNCH(0, k_scan_add, dim3((unsigned)n_blocks), dim3(kScanBlock), 0, s, w.tile_start,
+                       n_scan, w.block_sums);
+    if (n_atoms > 0) {
+        EMSIM_LAUNCH(0, k_bin_scatter, dim3(grid_for(n_atoms, 256)), dim3(256), 0, s, n_atoms,
+                           w.atom_tile, w.atom_local, w.tile_start, w.tile_count, w.bucket);
+    }
+    EMSIM_LAUNCH(0, k_bin_tiles, dim3((unsigned)n_tiles), dim3(kBinBlock), 0, s, w.bucket,
+                       w.tile_start, g, hist);
+    return EMSIM_HIP_OK;
+}
+
+extern "C" int emsim_hip_add_water(const emsim_hip_slice_builder* b, float* hist, int idx_h,
+                                   int idx_o, double voxel_water, uint64_t seed,
+                                   emsim_stream_t stream_) {
+    if (!b || !hist || idx_h < 0 || idx_o < 0 || idx_h >= b->n_elems || idx_o >= b->n_elems ||
+        idx_h == idx_o || !(voxel_water >= 0))
+        return fail(EMSIM_HIP_EINVAL, "add_water: bad arguments");
+    const int64_t n0 = b->one_slice ? 1 : b->n_slices;
+    const int64_t vox = n0 * b->n1 * b->n2;
+    const float lam_o = (float)voxel_water, lam_h = (float)(2.0 * voxel_water);
+    const float exp_o = (float)std::exp(-voxel_water), exp_h = (float)std::exp(-2.0 * voxel_water);
+    EMSIM_LAUNCH(0, k_add_water, dim3(grid_for(vox, 256)), dim3(256), 0, (hipStream_t)stream_,
+                       hist, b->n_elems, vox, idx_h, idx_o, lam_o, exp_o, lam_h, exp_h, seed);
+    return EMSIM_HIP_OK;
+}
+
+extern "C" size_t emsim_hip_slices_workspace_size(const emsim_hip_slice_builder* b,
+                                                  int max_chunk) {
+    if (!b) return 0;
+    int chunk = (max_chunk <= 0 || max_chunk > b->n_slices) ? b->n_slices : max_chunk;
+    return (size_t)b->n_elems * chunk * b->n1 * b->H * sizeof(float2);
+}
+
+extern "C" int emsim_hip_make_slices(const emsim_hip_slice_builder* b, const float* hist,
+                                     float* out, int symmetric_bandlimit, void* workspace,
+                                     size_t workspace_bytes, emsim_stream_t stream_) {
+    hipStream_t s = (hipStream_t)stream_;
+    if (!b || !hist || !out) return fail(EMSIM_HIP_EINVAL, "make_slices: bad arguments");
+    if (symmetric_bandlimit && b->n1 != b->n2)
+        return fail(EMSIM_HIP_EINVAL,
+                    "make_slices: the symmetric band-limit needs a square grid (the reference's "
+                    "mask does not broadcast for %dx%d)", b->n1, b->n2);
+    const size_t per_slice = (size_t)b->n_elems * b->n1 * b->H * sizeof(float2);
+    if (!workspace || workspace_bytes < per_slice)
+        return fail(EMSIM_HIP_EWORKSPACE, "make_slices: workspace needs at least %zu bytes",
+                    per_slice);
+    int chunk = (int)(workspace_bytes / per_slice);
+    if (chunk > b->n_slices) chunk = b->n_slices;
+    if (chunk > 65535) chunk = 65535;
+    float2* X = (float2*)workspace;
+    const int r = (b->n1 < b->n2 ? b->n1 : b->n2) / 2;
+    const float inv_npix = 1.0f / (float)((int64_t)b->n1 * b->n2);
+    for (int s0 = 0; s0 < b->n_slices; s0 += chunk) {
+        const int sc = (b->n_slices - s0) < chunk ? (b->n_slices - s0) : chunk;
+        int rc = r2c_pairs(b->n2, hist, X, (int64_t)b->n_elems * sc * (b->n1 / 2), b->n1, sc,
+                           b->n_slices, s0, b->tw, s);
+        if (rc) return rc;
+        rc = col_spectrum(b->n1, X, b->sf, b->n_elems, sc, b->H, b->n2,
+                          symmetric_bandlimit ? 1 : 0, r * r, inv_npix, b->tw, s);
+        if (rc) return rc;
+        rc = c2r_pairs(b->n2, X, out + (int64_t)s0 * b->n1 * b->n2, (int64_t)sc * (b->n1 / 2),
+                       1e-13f, b->tw, s);
+        if (rc) return rc;
+    }
+    return EMSIM_HIP_OK;
+}
+
+// ==========================================================================
+// wave propagator
+// ==========================================================================
+extern "C" int emsim_hip_wave_propagator_create(emsim_hip_wave_propagator** out, int n1, int n2,
+                                                float d1, float d2, float wave_length,
+                                                float relativity_gamma) {
+    if (!out) return fail(EMSIM_HIP_EINVAL, "wave_propagator_create: null output handle");
+    *out = nullptr;
+    if (!pow2_supported(n1) || !pow2_supported(n2))
+        return fail(EMSIM_HIP_EUNSUPPORTED,
+                    "wave_propagator_create: wave shape %dx%d; the hip backend implements "
+                    "powers of two from 8 to %d", n1, n2, kTwN);
+    if (!(d1 > 0) || !(d2 > 0) || !(wave_length > 0))
+        return fail(EMSIM_HIP_EINVAL, "wave_propagator_create: sizes must be positive");
+    emsim_hip_wave_propagator* w = new (std::nothrow) emsim_hip_wave_propagator();
+    if (!w) return fail(EMSIM_HIP_ENOMEM, "wave_propagator_create: out of host memory");
+    w->n1 = n1;
+    w->n2 = n2;
+    w->d1 = d1;
+    w->d2 = d2;
+    w->lam = wave_length;
+    w->gamma = relativity_gamma;
+    // WavePropagator.cpp:19-23
+    float dfx = 1.0f / d1;
+    w->dfx = dfx / (float)n1;
+    float dfy = 1.0f / d2;
+    w->dfy = dfy / (float)n2;
+    const float fmax = 0.5f / (d1 >= d2 ? d1 : d2);
+    const float filt = 0.6667f * fmax;
+    w->filter2 = filt * filt;
+    w->factor = wave_length * relativity_gamma;   // WavePropagator.cpp:42
+    int rc = get_twiddles(&w->tw);
+    if (rc) {
+        delete w;
+        return rc;
+    }
+    *out = w;
+    return EMSIM_HIP_OK;
+}
+
+extern "C" int emsim_hip_wave_propagator_destroy(emsim_hip_wave_propagator* w) {
+    delete w;
+    return EMSIM_HIP_OK;
+}
+
+extern "C" int emsim_hip_init_wave(const emsim_hip_wave_propagator* w, float value, float* wave,
+                                   int batch, emsim_stream_t stream_) {
+    if (!w || !wave || batch < 1) return fail(EMSIM_HIP_EINVAL, "init_wave: bad arguments");
+    const int64_t n = (int64_t)batch * w->n1 * w->n2;
+    EMSIM_LAUNCH(0, k_fill_complex, dim3(grid_for(n, 256)), dim3(256), 0,
+                       (hipStream_t)stream_, (float2*)wave, n, value);
+    return EMSIM_HIP_OK;
+}
+
+extern "C" int emsim_hip_slice_transmit(const emsim_hip_wave_propagator* w, const float* wave,
+                                        const float* slice, int64_t slice_batch_stride,
+                                        float* out, int batch, emsim_stream_t stream_) {
+    if (!w || !wave || !slice || !out || batch < 1 || slice_batch_stride < 0)
+        return fail(EMSIM_HIP_EINVAL, "slice_transmit: bad arguments");
+    const int64_t npix = (int64_t)w->n1 * w->n2;
+    const int64_t n = npix * batch;
+    EMSIM_LAUNCH(0, k_transmit, dim3(grid_for(n, 256)), dim3(256), 0, (hipStream_t)stream_,
+                       (const float2*)wave, slice, (float2*)out, npix, n, slice_batch_stride,
+                       w->factor);
+    return EMSIM_HIP_OK;
+}
+
+extern "C" int emsim_hip_space_propagate(const emsim_hip_wave_propagator* w, const float* wave,
+                                         float dz, float* out, int batch,
+                                         emsim_stream_t stream_) {
+    hipStream_t s = (hipStream_t)stream_;
+    if (!w || !wave || !out || batch < 1) return fail(EMSIM_HIP_EINVAL, "space_propagate: bad arguments");
+    if (batch > 65535) return fail(EMSIM_HIP_EUNSUPPORTED, "space_propagate: batch <= 65535");
+    const int64_t rows = (int64_t)batch * w->n1;
+    const int64_t npix = (int64_t)w->n1 * w->n2;
+    float2* o = (float2*)out;
+    int rc = row_fft(w->n2, false, (const float2*)wave, o, rows, w->tw, s);
+    if (!rc) rc = col_prop(w->n1, o, w->n2, w->n2, npix, batch, make_propagator(w, dz), w->tw, s);
+    if (!rc) rc = row_fft(w->n2, true, o, o, rows, w->tw, s);
+    return rc;
+}
+
+extern "C" int emsim_hip_multislice_propagate(const emsim_hip_wave_propagator* w,
+                                              const float* wave, const float* slices,
+                                              int n_slices, int64_t slices_batch_stride,
+                                              float dz, float* out, int batch,
+                                              emsim_stream_t stream_) {
+    hipStream_t s = (hipStream_t)stream_;
+    if (!w || !wave || !slices || !out || batch < 1 || n_slices < 1 || slices_batch_stride < 0)
+        return fail(EMSIM_HIP_EINVAL, "multislice_propagate: bad arguments");
+    if (batch > 65535) return fail(EMSIM_HIP_EUNSUPPORTED, "multislice_propagate: batch <= 65535");
+    const int64_t rows = (int64_t)batch * w->n1;
+    const int64_t npix = (int64_t)w->n1 * w->n2;
+    const PropagatorMul prop = make_propagator(w, dz);
+    float2* o = (float2*)out;
+    for (int sl = 0; sl < n_slices; ++sl) {
+        const float2* src = sl == 0 ? (const float2*)wave : o;
+        int rc = row_transmit_fft(w->n2, sl == 0, src, slices + (int64_t)sl * npix, o, rows,
+                                  w->n1, slices_batch_stride, w->factor, w->tw, s);
+        if (rc) return rc;
+        rc = col_prop(w->n1, o, w->n2, w->n2, npix, batch, prop, w->tw, s);
+        if (rc) return rc;
+    }
+    return row_fft(w->n2, true, o, o, rows, w->tw, s);
+}
+
+extern "C" int emsim_hip_lens_propagate(const emsim_hip_wave_propagator* w, const float* wave,
+                                        float cs_mm, float defocus, float aperture, float* out,
+                                        int batch, emsim_stream_t stream_) {
+    hipStream_t s = (hipStream_t)stream_;
+    if (!w || !wave || !out || batch < 1) return fail(EMSIM_HIP_EINVAL, "lens_propagate: bad arguments");
+    if (batch > 65535) return fail(EMSIM_HIP_EUNSUPPORTED, "lens_propagate: batch <= 65535");
+    const int64_t rows = (int64_t)batch * w->n1;
+    const int64_t npix = (int64_t)w->n1 * w->n2;
+    float2* o = (float2*)out;
+    int rc = row_fft(w->n2, false, (const float2*)wave, o, rows, w->tw, s);
+    if (!rc)
+        rc = col_lens(w->n1, o, w->n2, w->n2, npix, batch, make_lens(w, cs_mm, defocus, aperture),
+                      w->tw, s);
+    if (!rc) rc = row_fft(w->n2, true, o, o, rows, w->tw, s);
+    return rc;
+}
+
+extern "C" int emsim_hip_lens_intensity(const emsim_hip_wave_propagator* w, const float* wave,
+                                        float cs_mm, float defocus, float aperture,
+                                        float* scratch, float* image, int batch,
+                                        emsim_stream_t stream_) {
+    hipStream_t s = (hipStream_t)stream_;
+    if (!w || !wave || !scratch || !image || batch < 1)
+        return fail(EMSIM_HIP_EINVAL, "lens_intensity: bad arguments");
+    if (batch > 65535) return fail(EMSIM_HIP_EUNSUPPORTED, "lens_intensity: batch <= 65535");
+    const int64_t rows = (int64_t)batch * w->n1;
+    const int64_t npix = (int64_t)w->n1 * w->n2;
+    float2* o = (float2*)scratch;
+    int rc = row_fft(w->n2, false, (const float2*)wave, o, rows, w->tw, s);
+    if (!rc)
+        rc = col_lens(w->n1, o, w->n2, w->n2, npix, batch, make_lens(w, cs_mm, defocus, aperture),
+                      w->tw, s);
+    if (!rc) rc = row_ifft_intensity(w->n2, o, image, rows, w->tw, s);
+    return rc;
+}
+
+extern "C" int emsim_hip_intensity(const float* wave, float* out, int64_t n,
+                                   emsim_stream_t stream_) {
+    if (!wave || !out || n < 0) return fail(EMSIM_HIP_EINVAL, "intensity: bad arguments");
+    if (n == 0) return EMSIM_HIP_OK;
+    EMSIM_LAUNCH(0, k_intensity, dim3(grid_for(n, 256)), dim3(256), 0, (hipStream_t)stream_,
+                       (const float2*)wave, out, n);
+    return EMSIM_HIP_OK;
+}
+
+// ==========================================================================
+// per-kernel timing
+// ==========================================================================
+extern "C" int emsim_hip_timing_enable(int on) {
+    std::lock_guard<std::mutex> lock(g_timing_mutex);
+    g_timing_on = on != 0;
+    return EMSIM_HIP_OK;
+}
+
+extern "C" int emsim_hip_timing_collect(char* buf, size_t cap, size_t* needed) {
+    struct Agg {
+        std::string name;
+        int n;
+        long count;
+        double total, lo, hi;
+    };
+    std::vector<Agg> aggs;
+    std::vector<TimingRecord> recs;
+    {
+        std::lock_guard<std::mutex> lock(g_timing_mutex);
+        recs.swap(g_timing_records);
+    }
+    int rc = EMSIM_HIP_OK;
+    for (const TimingRecord& r : recs) {
+        float ms = 0.f;
+        hipError_t err = hipEventSynchronize(r.stop);
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, r.start, r.stop);
+        if (err != hipSuccess) {
+            rc = fail(EMSIM_HIP_EDEVICE, "timing_collect: %s", hipGetErrorString(err));
+            continue;
+        }
+        Agg* a = nullptr;
+        for (Agg& x : aggs)
+            if (x.n == r.n && x.name == r.name) a = &x;
+        if (!a) {
+            aggs.push_back({r.name, r.n, 0, 0.0, 1e30, 0.0});
+            a = &aggs.back();
+        }
+        a->count += 1;
+        a->total += ms;
+        a->lo = ms < a->lo ? ms : a->lo;
+        a->hi = ms > a->hi ? ms : a->hi;
+    }
+    {
+        std::lock_guard<std::mutex> lock(g_timing_mutex);
+        for (const TimingRecord& r : recs) {
+            g_event_pool.push_back(r.start);
+            g_event_pool.push_back(r.stop);
+        }
+    }
+    std::string text;
+    char line[256];
+    for (const Agg& a : aggs) {
+        snprintf(line, sizeof(line), "%s\t%d\t%ld\t%.6f\t%.6f\t%.6f\n", a.name.c_str(), a.n,
+                 a.count, a.total, a.lo, a.hi);
+        text += line;
+    }
+    if (needed) *needed = text.size() + 1;
+    if (buf && cap > 0) {
+        const size_t m = text.size() < cap - 1 ? text.size() : cap - 1;
+        memcpy(buf, text.data(), m);
+        buf[m] = '\0';
+    }
+    return rc;
+}

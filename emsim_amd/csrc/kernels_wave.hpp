@@ -1,0 +1,249 @@
+// Wave kernels: row passes, column passes and pointwise ops for the
+// multislice recurrence psi <- F^-1[ P . F[ t(V) psi ] ] / nPix
+// (reference: emsim/backend/fftw_ext/src/WavePropagator.cpp:41-138).
+//
+// A 2-D transform is split into a row pass (contiguous axis, one N/8-thread
+// group per row) and a column pass (tiles of C adjacent columns so that each
+// global access is a C*8-byte contiguous segment).  The multislice step is
+// fused into exactly two sweeps over the wave per slice:
+//   pass A (rows):    b -> IFFT_row -> * exp(i sigma V) -> FFT_row -> a
+//   pass B (columns): a -> FFT_col -> * P / nPix (masked) -> IFFT_col -> b
+// so a slice costs 8+8 (pass A) + 4 (V) + 8+8 (pass B) = 36 bytes per pixel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fft_lds.hpp"
+
+namespace emsim {
+
+// ---------------------------------------------------------------------------
+// frequency helpers -- float32 arithmetic in the reference's operation order,
+// never contracted into FMAs, so mask and aperture edges match exactly.
+// ---------------------------------------------------------------------------
+
+// Signed frequency of FFT bin i: the reference's ifftshift map
+// is = i + (i < (n+1)/2 ? n/2 : -(n+1)/2), then is - floor(n/2)
+// (WavePropagator.cpp:66-70).
+__device__ __forceinline__ int freq_int(int i, int n) {
+    const int is = i + (i < (n + 1) / 2 ? n / 2 : -((n + 1) / 2));
+    return is - n / 2;
+}
+
+__device__ __forceinline__ float freq2(int i, int j, int n1, int n2, float dfx, float dfy) {
+    const float fx = __fmul_rn((float)freq_int(i, n1), dfx);
+    const float fy = __fmul_rn((float)freq_int(j, n2), dfy);
+    return __fadd_rn(__fmul_rn(fx, fx), __fmul_rn(fy, fy));
+}
+
+// Fresnel propagator with the 2/3 mask and the 1/nPix of the unnormalised
+// inverse (WavePropagator.cpp:62-85).
+struct PropagatorMul {
+    int n1, n2;
+    float dfx, dfy;
+    float filter2;    // (0.6667 fmax)^2
+    float lpd;        // (lambda * pi) * dz
+    float inv_npix;   // 1/nPix, exact for power-of-two sizes
+    __device__ __forceinline__ float2 operator()(float2 w, int i, int j) const {
+        const float f2 = freq2(i, j, n1, n2, dfx, dfy);
+        if (!(f2 <= filter2)) return make_float2(0.f, 0.f);
+        float s, c;
+        sincosf(__fmul_rn(lpd, f2), &s, &c);
+        // (w.x + i w.y) (c - i s)
+        return make_float2((w.x * c + w.y * s) * inv_npix, (w.y * c - w.x * s) * inv_npix);
+    }
+};
+
+// Lens transfer function exp(-i chi) with the aperture (WavePropagator.cpp:105-135).
+struct LensMul {
+    int n1, n2;
+    float dfx, dfy;
+    float faper2;     // (aperture / lambda)^2
+    float c1, c2;     // chi = c1 f^4 - c2 f^2
+    float inv_npix;
+    __device__ __forceinline__ float2 operator()(float2 w, int i, int j) const {
+        const float f2 = freq2(i, j, n1, n2, dfx, dfy);
+        if (!(f2 <= faper2)) return make_float2(0.f, 0.f);
+        const float chi = __fsub_rn(__fmul_rn(__fmul_rn(c1, f2), f2), __fmul_rn(c2, f2));
+        float s, c;
+        sincosf(chi, &s, &c);
+        return make_float2((w.x * c + w.y * s) * inv_npix, (w.y * c - w.x * s) * inv_npix);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// launch geometry
+// ---------------------------------------------------------------------------
+template <int N>
+struct RowCfg {
+    static constexpr int T = N / kEPT;                       // threads per row
+    static constexpr int ROWS = T >= 256 ? 1 : 256 / T;      // rows per block
+    static constexpr int BLOCK = T * ROWS;
+    static constexpr int WORDS = LdsLayout<N>::kWords;       // floats per row plane
+};
+
+template <int N>
+struct ColCfg {
+    static constexpr int T = N / kEPT;                       // threads per column
+    // columns per tile; blocks stay at <= 512 threads so that a thread may use
+    // up to 256 VGPRs without spilling
+    static constexpr int C = (512 / T) < 1 ? 1 : ((512 / T) < 16 ? (512 / T) : 16);
+    static constexpr int BLOCK = T * C;
+    static constexpr int WORDS = LdsLayout<N>::kWords;
+};
+
+// ---------------------------------------------------------------------------
+// row kernels
+// ---------------------------------------------------------------------------
+#define EMSIM_ROW_PROLOGUE(N)                                                   \
+    using Cfg = RowCfg<N>;                                                      \
+    __shared__ float lds_re[Cfg::ROWS * Cfg::WORDS];                            \
+    __shared__ float lds_im[Cfg::ROWS * Cfg::WORDS];                            \
+    const int rib = threadIdx.x / Cfg::T;                                       \
+    const int t = threadIdx.x % Cfg::T;                                         \
+    const int64_t row = (int64_t)blockIdx.x * Cfg::ROWS + rib;                  \
+    const bool valid = row < n_rows;                                            \
+    float* re = lds_re + rib * Cfg::WORDS;                                      \
+    float* im = lds_im + rib * Cfg::WORDS;
+
+// Plain row transform of n_rows rows of length N (in place allowed).
+template <int N, bool INV>
+__global__ __launch_bounds__(RowCfg<N>::BLOCK) void k_row_fft(const float2* in, float2* out,
+                                                               int64_t n_rows,
+                                                               const float2* __restrict__ tw) {
+    EMSIM_ROW_PROLOGUE(N)
+    float2 v[kEPT];
+    const float2* src = in + row * N;
+#pragma unroll
+    for (int m = 0; m < kEPT; ++m) v[m] = valid ? src[t + m * Cfg::T] : make_float2(0.f, 0.f);
+    fft_n<N, INV>(v, re, im, t, tw);
+    if (valid) {
+        float2* dst = out + row * N;
+#pragma unroll
+        for (int m = 0; m < kEPT; ++m) dst[t + m * Cfg::T] = v[m];
+    }
+}
+
+// Multislice pass A: [IFFT_row] -> * (cos, sin)(factor V) -> FFT_row.
+// FIRST: the input is the real-space wave itself (no leading IFFT).
+// Rows belong to images of rows_per_img rows; the slice row of global row r
+// is V + (r / rows_per_img) * v_img_stride + (r % rows_per_img) * N.
+template <int N, bool FIRST>
+__global__ __launch_bounds__(RowCfg<N>::BLOCK) void k_row_transmit_fft(
+    const float2* in, const float* __restrict__ V, float2* out, int64_t n_rows,
+    int rows_per_img, int64_t v_img_stride, float factor, const float2* __restrict__ tw) {
+    EMSIM_ROW_PROLOGUE(N)
+    float2 v[kEPT];
+    const float2* src = in + row * N;
+#pragma unroll
+    for (int m = 0; m < kEPT; ++m) v[m] = valid ? src[t + m * Cfg::T] : make_float2(0.f, 0.f);
+    if constexpr (!FIRST) fft_n<N, true>(v, re, im, t, tw);
+    if (valid) {
+        const int64_t img = row / rows_per_img;
+        const float* vrow = V + img * v_img_stride + (row - img * rows_per_img) * N;
+#pragma unroll
+        for (int m = 0; m < kEPT; ++m) {
+            float s, c;
+            sincosf(__fmul_rn(vrow[t + m * Cfg::T], factor), &s, &c);
+            const float2 w = v[m];
+            v[m] = make_float2(w.x * c - w.y * s, w.x * s + w.y * c);
+        }
+    }
+    fft_n<N, false>(v, re, im, t, tw);
+    if (valid) {
+        float2* dst = out + row * N;
+#pragma unroll
+        for (int m = 0; m < kEPT; ++m) dst[t + m * Cfg::T] = v[m];
+    }
+}
+
+// IFFT_row followed by |psi|^2 (the lens epilogue; the column pass already
+// applied H / nPix).
+template <int N>
+__global__ __launch_bounds__(RowCfg<N>::BLOCK) void k_row_ifft_intensity(
+    const float2* in, float* __restrict__ img_out, int64_t n_rows, const float2* __restrict__ tw) {
+    EMSIM_ROW_PROLOGUE(N)
+    float2 v[kEPT];
+    const float2* src = in + row * N;
+#pragma unroll
+    for (int m = 0; m < kEPT; ++m) v[m] = valid ? src[t + m * Cfg::T] : make_float2(0.f, 0.f);
+    fft_n<N, true>(v, re, im, t, tw);
+    if (valid) {
+        float* dst = img_out + row * N;
+#pragma unroll
+        for (int m = 0; m < kEPT; ++m) dst[t + m * Cfg::T] = v[m].x * v[m].x + v[m].y * v[m].y;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// column kernel: FFT_col -> pointwise Mul(k1, j) -> IFFT_col, in place.
+// grid.x = tiles of C columns, grid.y = images; data is n_img images of
+// N rows x ld complex, image stride img_stride elements; ncols <= ld columns
+// are transformed.
+// ---------------------------------------------------------------------------
+template <int N, class Mul>
+__global__ __launch_bounds__(ColCfg<N>::BLOCK) void k_col_fft_mul_ifft(
+    float2* data, int ld, int ncols, int64_t img_stride, Mul mul, const float2* __restrict__ tw) {
+    using Cfg = ColCfg<N>;
+    __shared__ float lds_re[Cfg::C * Cfg::WORDS];
+    __shared__ float lds_im[Cfg::C * Cfg::WORDS];
+    const int c = threadIdx.x % Cfg::C;
+    const int t = threadIdx.x / Cfg::C;
+    const int col = blockIdx.x * Cfg::C + c;
+    const bool valid = col < ncols;
+    float2* base = data + (int64_t)blockIdx.y * img_stride + col;
+    float* re = lds_re + c * Cfg::WORDS;
+    float* im = lds_im + c * Cfg::WORDS;
+
+    float2 v[kEPT];
+#pragma unroll
+    for (int m = 0; m < kEPT; ++m)
+        v[m] = valid ? base[(int64_t)(t + m * Cfg::T) * ld] : make_float2(0.f, 0.f);
+    fft_n<N, false>(v, re, im, t, tw);
+#pragma unroll
+    for (int m = 0; m < kEPT; ++m) v[m] = mul(v[m], t + m * Cfg::T, col);
+    fft_n<N, true>(v, re, im, t, tw);
+    if (valid) {
+#pragma unroll
+        for (int m = 0; m < kEPT; ++m) base[(int64_t)(t + m * Cfg::T) * ld] = v[m];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// pointwise kernels
+// ---------------------------------------------------------------------------
+__global__ void k_fill_complex(float2* out, int64_t n, float value) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        out[i] = make_float2(value, 0.f);
+}
+
+__global__ void k_fill_u32(uint32_t* out, int64_t n, uint32_t value) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        out[i] = value;
+}
+
+// WavePropagator.cpp:41-55: psi * (cos(factor V), sin(factor V)).
+__global__ void k_transmit(const float2* in, const float* __restrict__ V, float2* out,
+                           int64_t npix, int64_t n_total, int64_t v_img_stride, float factor) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_total; i += stride) {
+        const int64_t img = i / npix;
+        const float pix = V[img * v_img_stride + (i - img * npix)];
+        float s, c;
+        sincosf(__fmul_rn(pix, factor), &s, &c);
+        const float2 w = in[i];
+        out[i] = make_float2(w.x * c - w.y * s, w.x * s + w.y * c);
+    }
+}
+
+__global__ void k_intensity(const float2* __restrict__ in, float* __restrict__ out, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const float2 w = in[i];
+        out[i] = w.x * w.x + w.y * w.y;
+    }
+}
+
+}  // namespace emsim

@@ -1,0 +1,177 @@
+/*
+ * emsim_hip.h -- C ABI of the MI355X (gfx950) multislice backend.
+ *
+ * Plain pointers and sizes only: every array argument named *_dev is device
+ * memory owned by the caller (the Python layer allocates it with torch's
+ * caching allocator); *_host arguments are host memory read during the call.
+ * All launches go to the caller's stream; nothing here synchronises the
+ * device except the create functions, which upload constant tables.
+ *
+ * Every function returns 0 on success or a negative EMSIM_HIP_E* code;
+ * emsim_hip_last_error() returns the message of the calling thread's most
+ * recent failure.
+ *
+ * Each entry point names the reference interface it replaces (paths are
+ * relative to the reference repository Varato/emsim).
+ */
+#ifndef EMSIM_HIP_H
+#define EMSIM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EMSIM_HIP_ABI_VERSION 1
+
+#define EMSIM_HIP_OK 0
+#define EMSIM_HIP_EINVAL -1     /* bad argument (shape, size, null pointer) */
+#define EMSIM_HIP_EUNSUPPORTED -2 /* size outside what the kernels implement */
+#define EMSIM_HIP_EWORKSPACE -3 /* workspace smaller than required */
+#define EMSIM_HIP_EDEVICE -4    /* HIP runtime error */
+#define EMSIM_HIP_ENOMEM -5     /* host or device allocation failed */
+
+typedef struct ihipStream_t* emsim_stream_t; /* == hipStream_t */
+
+/* ---- library ---------------------------------------------------------- */
+int emsim_hip_abi_version(void);
+const char* emsim_hip_last_error(void);
+/* 1 if n is a lateral size the FFT kernels implement (2^k, 8 <= n <= 4096). */
+int emsim_hip_fft_size_supported(int n);
+
+/* Host-only helper, exported for the binning tests: the float64 bin edges
+ * the reference builds with numpy.arange(s, s + n d + d, d), s = -d (n//2)
+ * (emsim/backend/slice_builder_base.py:180-186,202-208), and the float32
+ * table the kernels compare against (each edge rounded up to float32).
+ * Writes at most cap entries; *len_out = number of edges numpy would make. */
+int emsim_hip_bin_edges(int n, double d, double* edges64_host, float* edges32_host, int cap,
+                        int* len_out);
+
+/* ---- slice builder ---------------------------------------------------- */
+/* Replaces the fftw pybind class slice_kernel.MultiSlicesBuilder
+ * (emsim/backend/fftw_ext/src/slice_kernel.cpp:20-64, SliceBuilder.cpp:13-85)
+ * and the cuda OneSliceBuilder / MultiSlicesBuilder
+ * (emsim/backend/cuda_ext/src/slice_kernel_cuda.cpp:179-214).
+ * scattering_factors_host: (n_elems, n1, n2/2+1) float32, the table built by
+ * emsim/backend/slice_builder_base.py:30-33.  one_slice != 0 builds the 2-D
+ * (one-slice) variant: n_slices must be 1 and coordinate column 0 is ignored
+ * (slice_builder_base.py:202-225). */
+typedef struct emsim_hip_slice_builder emsim_hip_slice_builder;
+
+int emsim_hip_slice_builder_create(emsim_hip_slice_builder** out,
+                                   const float* scattering_factors_host, int n_elems,
+                                   int n_slices, int n1, int n2, double dz, double d1,
+                                   double d2, int one_slice);
+int emsim_hip_slice_builder_destroy(emsim_hip_slice_builder* b);
+
+/* Workspace bytes for binning n_atoms atoms. */
+size_t emsim_hip_bin_workspace_size(const emsim_hip_slice_builder* b, int64_t n_atoms);
+
+/* Replaces _bin_atoms_multi_slices / _bin_atoms_one_slice
+ * (slice_builder_base.py:180-225) and the cuda binAtoms*Kernel launchers
+ * (emsim/backend/cuda_ext/src/SliceBuilder_kernel.cu:290-317).
+ * coords_dev: (n_atoms, 3) float32 rows (z, x, y), sorted by element;
+ * counts_host: (n_elems) atoms per element, sum <= n_atoms.
+ * hist_dev: (n_elems, n_slices, n1, n2) float32 integer-valued counts
+ * ((n_elems, n1, n2) for a one-slice builder); numpy.histogramdd semantics
+ * on the reference's float64 edges, bit-exact. */
+int emsim_hip_bin_atoms(const emsim_hip_slice_builder* b, const float* coords_dev,
+                        const int64_t* counts_host, int64_t n_atoms, float* hist_dev,
+                        void* workspace_dev, size_t workspace_bytes, emsim_stream_t stream);
+
+/* Replaces MultiSlicesBuilderBase.add_water (slice_builder_base.py:134-147)
+ * and the cuda variant (emsim/backend/slice_builder_cuda.py:72-85): every
+ * voxel whose count is zero for all elements gets Poisson(voxel_water)
+ * oxygens and Poisson(2 voxel_water) hydrogens, added in place to the
+ * planes of elements idx_o and idx_h.  voxel_water = water_num_dens d1 d2 dz.
+ * The reference draws from numpy's unseeded global RNG; here the draws are
+ * a pure function of seed and voxel index. */
+int emsim_hip_add_water(const emsim_hip_slice_builder* b, float* hist_dev, int idx_h, int idx_o,
+                        double voxel_water, uint64_t seed, emsim_stream_t stream);
+
+/* Workspace bytes for make_slices when at most max_chunk slices are
+ * transformed per pass (max_chunk <= 0: all slices at once). */
+size_t emsim_hip_slices_workspace_size(const emsim_hip_slice_builder* b, int max_chunk);
+
+/* Replaces MultiSlicesBuilder::makeMultiSlices (SliceBuilder.cpp:27-85)
+ * followed by the numpy band-limit and clip (slice_builder_fftw.py:43-48),
+ * and make_one_slice (slice_builder_fftw.py:24-32).
+ * hist_dev: (n_elems, n_slices, n1, n2) float32; out_dev: (n_slices, n1, n2)
+ * float32 projected potential, clipped at 1e-13.  symmetric_bandlimit
+ * requires n1 == n2 (the reference's mask only broadcasts then). */
+int emsim_hip_make_slices(const emsim_hip_slice_builder* b, const float* hist_dev,
+                          float* out_dev, int symmetric_bandlimit, void* workspace_dev,
+                          size_t workspace_bytes, emsim_stream_t stream);
+
+/* ---- wave propagator -------------------------------------------------- */
+/* Replaces the fftw pybind class wave_kernel.WavePropagator
+ * (emsim/backend/fftw_ext/src/wave_kernel.cpp:19-114, WavePropagator.cpp)
+ * and the cuda one (emsim/backend/cuda_ext/src/wave_kernel_cuda.cpp:28-143).
+ * wave_length / relativity_gamma are the float32 values the reference's
+ * binding receives.  Every wave argument is (batch, n1, n2) complex64 stored
+ * as interleaved float pairs; in-place calls (out == in) are allowed. */
+typedef struct emsim_hip_wave_propagator emsim_hip_wave_propagator;
+
+int emsim_hip_wave_propagator_create(emsim_hip_wave_propagator** out, int n1, int n2,
+                                     float d1, float d2, float wave_length,
+                                     float relativity_gamma);
+int emsim_hip_wave_propagator_destroy(emsim_hip_wave_propagator* w);
+
+/* wave_propagator_base.py:25-29: every pixel = (value, 0). */
+int emsim_hip_init_wave(const emsim_hip_wave_propagator* w, float value, float* wave_dev,
+                        int batch, emsim_stream_t stream);
+
+/* WavePropagator::sliceTransmit (WavePropagator.cpp:41-55).
+ * slice_dev: (batch, n1, n2) float32, or (n1, n2) shared when
+ * slice_batch_stride == 0. */
+int emsim_hip_slice_transmit(const emsim_hip_wave_propagator* w, const float* wave_dev,
+                             const float* slice_dev, int64_t slice_batch_stride,
+                             float* out_dev, int batch, emsim_stream_t stream);
+
+/* WavePropagator::spacePropagate (WavePropagator.cpp:57-88). */
+int emsim_hip_space_propagate(const emsim_hip_wave_propagator* w, const float* wave_dev,
+                              float dz, float* out_dev, int batch, emsim_stream_t stream);
+
+/* WavePropagator::singleSlicePropagate / multiSlicePropagate
+ * (WavePropagator.cpp:90-103).  slices_dev: (batch, n_slices, n1, n2)
+ * float32, or one (n_slices, n1, n2) stack shared by the batch when
+ * slices_batch_stride == 0. */
+int emsim_hip_multislice_propagate(const emsim_hip_wave_propagator* w, const float* wave_dev,
+                                   const float* slices_dev, int n_slices,
+                                   int64_t slices_batch_stride, float dz, float* out_dev,
+                                   int batch, emsim_stream_t stream);
+
+/* WavePropagator::lensPropagate (WavePropagator.cpp:105-138). */
+int emsim_hip_lens_propagate(const emsim_hip_wave_propagator* w, const float* wave_dev,
+                             float cs_mm, float defocus, float aperture, float* out_dev,
+                             int batch, emsim_stream_t stream);
+
+/* lensPropagate followed by |psi|^2 (emsim/pipe.py:66-84), fused: the
+ * intensity is written without materialising the image wave.
+ * scratch_dev: (batch, n1, n2) complex64; it may be wave_dev itself, which
+ * is then overwritten.  image_dev: (batch, n1, n2) float32. */
+int emsim_hip_lens_intensity(const emsim_hip_wave_propagator* w, const float* wave_dev,
+                             float cs_mm, float defocus, float aperture, float* scratch_dev,
+                             float* image_dev, int batch, emsim_stream_t stream);
+
+/* |psi|^2 of n complex values (emsim/pipe.py:76,82). */
+int emsim_hip_intensity(const float* wave_dev, float* out_dev, int64_t n,
+                        emsim_stream_t stream);
+
+/* ---- instrumentation (no counterpart in the reference, which has no
+ * profiler; simulator.py:77,89-91 only logs wall time per image) ---------- */
+/* While enabled, every kernel launch is bracketed by two HIP events on its
+ * stream. */
+int emsim_hip_timing_enable(int on);
+/* Waits for the recorded events and writes one line per (kernel, length):
+ * "name\tn\tcount\ttotal_ms\tmin_ms\tmax_ms\n"; then forgets the records.
+ * *needed receives the buffer size the full text needs. */
+int emsim_hip_timing_collect(char* buf, size_t cap, size_t* needed);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EMSIM_HIP_H */
