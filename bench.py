@@ -38,7 +38,25 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-timing", action="store_true",
                     help="do not bracket launches with HIP events")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 4 device-to-device copies of MIB MiB (a known-size "
+                         "reference for HBM byte counters) and exit")
     return ap.parse_args()
+
+
+def calibrate_copy(mib: int):
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(4):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 4
+    print(json.dumps({"calibration": "copy", "bytes_read": n * 4, "bytes_written": n * 4,
+                      "GBs": round(2 * n * 4 / dt / 1e9, 1)}), flush=True)
 
 
 def workload(batch: int, n_atoms_range=(10000, 100000), n_base: int = 8, seed: int = 10000):
@@ -117,6 +135,9 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(images shard across GPUs with no collective; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
 
     import torch
     import emsim_amd

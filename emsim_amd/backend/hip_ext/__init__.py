@@ -12,11 +12,15 @@ built, as the reference's extension wrappers do
 (``emsim/backend/slice_builder_fftw.py:7-10``).
 """
 import ctypes
+import os
 from ctypes import (POINTER, c_char_p, c_double, c_float, c_int, c_int64, c_size_t, c_uint64,
                     c_void_p)
 from pathlib import Path
 
-LIB_PATH = Path(__file__).resolve().parent / "libemsim_hip.so"
+# EMSIM_HIP_LIB may name another build of the same library (e.g. one compiled
+# with different tile constants) to load instead of the in-tree one.
+LIB_PATH = Path(os.environ.get("EMSIM_HIP_LIB") or
+                Path(__file__).resolve().parent / "libemsim_hip.so").resolve()
 
 if not LIB_PATH.exists():
     raise ImportError(f"hip extension {LIB_PATH} cannot be found. Compile it first "

@@ -47,8 +47,11 @@ class _BuilderHandle:
 
     def __del__(self):
         h = getattr(self, "h", None)
-        if h is not None and h.value:
-            hip_ext.raw("emsim_hip_slice_builder_destroy")(h)
+        if h is not None and h.value and hip_ext is not None:
+            try:
+                hip_ext.raw("emsim_hip_slice_builder_destroy")(h)
+            except Exception:       # interpreter shutdown
+                pass
             self.h = None
 
     # -- operations --------------------------------------------------------

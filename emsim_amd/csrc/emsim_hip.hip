@@ -173,6 +173,10 @@ float round_up_f32(double x) {
 
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Row stride of the half-spectrum workspace: H = n2/2+1 rounded up to a
+// multiple of 16 complex, so that column tiles start on 64-byte boundaries.
+int half_ld(int H) { return (H + 15) & ~15; }
+
 int grid_for(int64_t n, int block) {
     int64_t g = ceil_div(n, block);
     if (g < 1) g = 1;
@@ -282,32 +286,32 @@ int launch_col_lens(float2* data, int ld, int ncols, int64_t img_stride, int n_i
 
 template <int N>
 int launch_r2c_pairs(const float* in, float2* out, int64_t n_pairs, int rows_per_plane,
-                     int planes_per_group, int64_t in_group_stride, int64_t in_plane0,
+                     int planes_per_group, int64_t in_group_stride, int64_t in_plane0, int ldh,
                      const float2* tw, hipStream_t s) {
     using Cfg = RowCfg<N>;
     EMSIM_LAUNCH(N, (k_row_r2c_pairs<N>), dim3((unsigned)ceil_div(n_pairs, Cfg::ROWS)),
                        dim3(Cfg::BLOCK), 0, s, in, out, n_pairs, rows_per_plane, planes_per_group,
-                       in_group_stride, in_plane0, tw);
+                       in_group_stride, in_plane0, ldh, tw);
     return EMSIM_HIP_OK;
 }
 
 template <int N>
-int launch_c2r_pairs(const float2* in, float* out, int64_t n_pairs, float clip_lo,
+int launch_c2r_pairs(const float2* in, float* out, int64_t n_pairs, float clip_lo, int ldh,
                      const float2* tw, hipStream_t s) {
     using Cfg = RowCfg<N>;
     EMSIM_LAUNCH(N, (k_row_c2r_pairs<N>), dim3((unsigned)ceil_div(n_pairs, Cfg::ROWS)),
-                       dim3(Cfg::BLOCK), 0, s, in, out, n_pairs, clip_lo, tw);
+                       dim3(Cfg::BLOCK), 0, s, in, out, n_pairs, clip_lo, ldh, tw);
     return EMSIM_HIP_OK;
 }
 
 template <int N>
-int launch_col_spectrum(float2* X, const float* sf, int n_elems, int n_sc, int H, int n2,
-                        int bandlimit, int band_r2, float inv_npix, const float2* tw,
+int launch_col_spectrum(float2* X, const float* sf, int n_elems, int n_sc, int H, int ldh,
+                        int n2, int bandlimit, int band_r2, float inv_npix, const float2* tw,
                         hipStream_t s) {
-    using Cfg = ColCfg<N>;
+    using Cfg = SpecCfg<N>;
     const dim3 grid((unsigned)ceil_div(H, Cfg::C), (unsigned)n_sc);
     EMSIM_LAUNCH(N, (k_col_spectrum<N>), grid, dim3(Cfg::BLOCK), 0, s, X, sf, n_elems, n_sc, H,
-                       n2, bandlimit, band_r2, inv_npix, tw);
+                       ldh, n2, bandlimit, band_r2, inv_npix, tw);
     return EMSIM_HIP_OK;
 }
 
@@ -334,18 +338,19 @@ int col_lens(int n, float2* data, int ld, int ncols, int64_t img_stride, int n_i
     EMSIM_DISPATCH(n, launch_col_lens, data, ld, ncols, img_stride, n_img, mul, tw, s);
 }
 int r2c_pairs(int n, const float* in, float2* out, int64_t n_pairs, int rows_per_plane,
-              int planes_per_group, int64_t in_group_stride, int64_t in_plane0, const float2* tw,
-              hipStream_t s) {
-    EMSIM_DISPATCH(n, launch_r2c_pairs, in, out, n_pairs, rows_per_plane, planes_per_group,
-                   in_group_stride, in_plane0, tw, s);
-}
-int c2r_pairs(int n, const float2* in, float* out, int64_t n_pairs, float clip_lo,
+              int planes_per_group, int64_t in_group_stride, int64_t in_plane0, int ldh,
               const float2* tw, hipStream_t s) {
-    EMSIM_DISPATCH(n, launch_c2r_pairs, in, out, n_pairs, clip_lo, tw, s);
+    EMSIM_DISPATCH(n, launch_r2c_pairs, in, out, n_pairs, rows_per_plane, planes_per_group,
+                   in_group_stride, in_plane0, ldh, tw, s);
 }
-int col_spectrum(int n, float2* X, const float* sf, int n_elems, int n_sc, int H, int n2,
-                 int bandlimit, int band_r2, float inv_npix, const float2* tw, hipStream_t s) {
-    EMSIM_DISPATCH(n, launch_col_spectrum, X, sf, n_elems, n_sc, H, n2, bandlimit, band_r2,
+int c2r_pairs(int n, const float2* in, float* out, int64_t n_pairs, float clip_lo, int ldh,
+              const float2* tw, hipStream_t s) {
+    EMSIM_DISPATCH(n, launch_c2r_pairs, in, out, n_pairs, clip_lo, ldh, tw, s);
+}
+int col_spectrum(int n, float2* X, const float* sf, int n_elems, int n_sc, int H, int ldh,
+                 int n2, int bandlimit, int band_r2, float inv_npix, const float2* tw,
+                 hipStream_t s) {
+    EMSIM_DISPATCH(n, launch_col_spectrum, X, sf, n_elems, n_sc, H, ldh, n2, bandlimit, band_r2,
                    inv_npix, tw, s);
 }
 
@@ -629,7 +634,7 @@ extern "C" size_t emsim_hip_slices_workspace_size(const emsim_hip_slice_builder*
                                                   int max_chunk) {
     if (!b) return 0;
     int chunk = (max_chunk <= 0 || max_chunk > b->n_slices) ? b->n_slices : max_chunk;
-    return (size_t)b->n_elems * chunk * b->n1 * b->H * sizeof(float2);
+    return (size_t)b->n_elems * chunk * b->n1 * half_ld(b->H) * sizeof(float2);
 }
 
 extern "C" int emsim_hip_make_slices(const emsim_hip_slice_builder* b, const float* hist,
@@ -641,7 +646,8 @@ extern "C" int emsim_hip_make_slices(const emsim_hip_slice_builder* b, const flo
         return fail(EMSIM_HIP_EINVAL,
                     "make_slices: the symmetric band-limit needs a square grid (the reference's "
                     "mask does not broadcast for %dx%d)", b->n1, b->n2);
-    const size_t per_slice = (size_t)b->n_elems * b->n1 * b->H * sizeof(float2);
+    const int ldh = half_ld(b->H);
+    const size_t per_slice = (size_t)b->n_elems * b->n1 * ldh * sizeof(float2);
     if (!workspace || workspace_bytes < per_slice)
         return fail(EMSIM_HIP_EWORKSPACE, "make_slices: workspace needs at least %zu bytes",
                     per_slice);
@@ -654,13 +660,13 @@ extern "C" int emsim_hip_make_slices(const emsim_hip_slice_builder* b, const flo
     for (int s0 = 0; s0 < b->n_slices; s0 += chunk) {
         const int sc = (b->n_slices - s0) < chunk ? (b->n_slices - s0) : chunk;
         int rc = r2c_pairs(b->n2, hist, X, (int64_t)b->n_elems * sc * (b->n1 / 2), b->n1, sc,
-                           b->n_slices, s0, b->tw, s);
+                           b->n_slices, s0, ldh, b->tw, s);
         if (rc) return rc;
-        rc = col_spectrum(b->n1, X, b->sf, b->n_elems, sc, b->H, b->n2,
+        rc = col_spectrum(b->n1, X, b->sf, b->n_elems, sc, b->H, ldh, b->n2,
                           symmetric_bandlimit ? 1 : 0, r * r, inv_npix, b->tw, s);
         if (rc) return rc;
         rc = c2r_pairs(b->n2, X, out + (int64_t)s0 * b->n1 * b->n2, (int64_t)sc * (b->n1 / 2),
-                       1e-13f, b->tw, s);
+                       1e-13f, ldh, b->tw, s);
         if (rc) return rc;
     }
     return EMSIM_HIP_OK;

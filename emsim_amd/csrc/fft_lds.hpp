@@ -1,14 +1,14 @@
 // LDS-staged Stockham FFT building blocks for CDNA4 (gfx950), complex64.
 //
-// A length-N transform (N = 2^k, 8 <= N <= 4096) is computed by N/8 threads
-// that each hold 8 complex values in registers.  Thread t loads elements
-// x[t + m*N/8] (m = 0..7), runs a leading radix-2/4/8 pass followed by
-// radix-8 passes (autosorting Stockham, Govindaraju et al. 2008), exchanging
-// through LDS between passes, and ends holding X[t + m*N/8] in the same
-// register order it loaded from.  That symmetry is what lets the row and
-// column kernels fuse pointwise work (transmission, propagator, form-factor
-// sum, |psi|^2) between a forward and an inverse transform without another
-// trip through LDS or HBM.
+// A length-N transform (N = 2^k, 8 <= N <= 4096) is computed by N/EPT
+// threads that each hold EPT complex values in registers (EPT = 8, 16 or
+// 32).  Thread t loads elements x[t + m*N/EPT] (m = 0..EPT-1), runs a
+// leading radix-2/4/8 pass followed by radix-8 passes (autosorting Stockham,
+// Govindaraju et al. 2008), exchanging through LDS between passes, and ends
+// holding X[t + m*N/EPT] in the same register order it loaded from.  That
+// symmetry is what lets the row and column kernels fuse pointwise work
+// (transmission, propagator, form-factor sum, |psi|^2) between a forward and
+// an inverse transform without another trip through LDS or HBM.
 //
 // Twiddles come from one table w[m] = exp(-2 pi i m / 4096) that the host
 // computes in float64 and rounds to float32; a pass of radix R over
@@ -20,14 +20,12 @@
 namespace emsim {
 
 constexpr int kTwN = 4096;   // twiddle table length == largest supported N
-constexpr int kEPT = 8;      // complex elements per thread
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
-__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 
 // a * w (forward) or a * conj(w) (inverse)
 template <bool INV>
@@ -93,24 +91,28 @@ __device__ __forceinline__ void dft(float2* a) {
 
 constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
 
-// LDS row layout.  Elements are stored as two float planes (re, im) with one
+// LDS plane layout of one transform: separate re / im float planes with one
 // pad word every 32 so that the power-of-two strided Stockham writes spread
-// over the LDS banks.
+// over the banks.  kStride (odd) is the distance between the planes of
+// neighbouring transforms, so that C side-by-side transforms (the column
+// kernels) do not fall on the same banks.
 template <int N>
 struct LdsLayout {
     static constexpr int kWords = N + N / 32;
+    static constexpr int kStride = kWords | 1;
     __device__ static __forceinline__ int at(int i) { return i + (i >> 5); }
 };
 
-// One Stockham pass: radix R over sub-transforms of length Ns.
-// Before: v[m] = x[t + m N/8].  Butterfly j = t + q N/8 reads x[j + r N/R]
-// (= v[q + r 8/R]) and writes y[(j/Ns) Ns R + j%Ns + r Ns].
-template <int N, int R, int Ns, bool INV>
-__device__ __forceinline__ void stockham_pass(float2 (&v)[kEPT], float* __restrict__ re,
+// One Stockham pass: radix R over sub-transforms of length Ns, EPT values
+// per thread.  Before: v[m] = x[t + m N/EPT].  Butterfly j = t + q N/EPT
+// (q < EPT/R) reads x[j + r N/R] (= v[q + r EPT/R]) and writes
+// y[(j/Ns) Ns R + j%Ns + r Ns].
+template <int N, int EPT, int R, int Ns, bool INV>
+__device__ __forceinline__ void stockham_pass(float2 (&v)[EPT], float* __restrict__ re,
                                               float* __restrict__ im, int t,
                                               const float2* __restrict__ tw) {
-    constexpr int NB = kEPT / R;
-    constexpr int S = N / kEPT;
+    constexpr int NB = EPT / R;
+    constexpr int S = N / EPT;
     constexpr bool LAST = (Ns * R == N);
     using L = LdsLayout<N>;
 #pragma unroll
@@ -141,7 +143,7 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[kEPT], float* __restri
     if constexpr (!LAST) {
         __syncthreads();
 #pragma unroll
-        for (int m = 0; m < kEPT; ++m) {
+        for (int m = 0; m < EPT; ++m) {
             const int idx = L::at(t + m * S);
             v[m] = make_float2(re[idx], im[idx]);
         }
@@ -149,26 +151,74 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[kEPT], float* __restri
     }
 }
 
-template <int N, int Ns, bool INV>
-__device__ __forceinline__ void fft_rest(float2 (&v)[kEPT], float* re, float* im, int t,
+template <int N, int EPT, int Ns, bool INV>
+__device__ __forceinline__ void fft_rest(float2 (&v)[EPT], float* re, float* im, int t,
                                          const float2* __restrict__ tw) {
     if constexpr (Ns < N) {
-        stockham_pass<N, 8, Ns, INV>(v, re, im, t, tw);
-        fft_rest<N, Ns * 8, INV>(v, re, im, t, tw);
+        stockham_pass<N, EPT, 8, Ns, INV>(v, re, im, t, tw);
+        fft_rest<N, EPT, Ns * 8, INV>(v, re, im, t, tw);
     }
 }
 
-// Unnormalised length-N DFT of the values held by the N/8 threads of one
+// Unnormalised length-N DFT of the values held by the N/EPT threads of one
 // row/column group.  Every thread of the block must call it (it contains
 // block barriers when N > 8).
-template <int N, bool INV>
-__device__ __forceinline__ void fft_n(float2 (&v)[kEPT], float* re, float* im, int t,
+template <int N, int EPT, bool INV>
+__device__ __forceinline__ void fft_n(float2 (&v)[EPT], float* re, float* im, int t,
                                       const float2* __restrict__ tw) {
     static_assert(N >= 8 && N <= kTwN && (N & (N - 1)) == 0, "unsupported FFT length");
+    static_assert(EPT >= 8 && EPT % 8 == 0 && EPT <= N, "EPT must be a multiple of 8");
     constexpr int L = ilog2(N);
     constexpr int R0 = (L % 3 == 1) ? 2 : (L % 3 == 2) ? 4 : 8;
-    stockham_pass<N, R0, 1, INV>(v, re, im, t, tw);
-    fft_rest<N, R0, INV>(v, re, im, t, tw);
+    stockham_pass<N, EPT, R0, 1, INV>(v, re, im, t, tw);
+    fft_rest<N, EPT, R0, INV>(v, re, im, t, tw);
+}
+
+// ---------------------------------------------------------------------------
+// sin and cos of a float32 argument.
+// |x| <= 8192: Cody-Waite reduction by pi/2 with a three-part float32 split
+// of pi/2 (products exact inside FMA), then Taylor polynomials on
+// [-pi/4, pi/4] (sin to x^11, cos to x^12; truncation error < 2e-10, well
+// under float32 rounding), then quadrant selection.  Larger arguments use the
+// ROCm device library's sincosf.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void fast_sincos(float x, float* s_out, float* c_out) {
+    if (!(fabsf(x) <= 8192.f)) {
+        sincosf(x, s_out, c_out);
+        return;
+    }
+    // pi/2 = P1 + P2 + P3 (P1, P2 float32 with trailing zero bits)
+    const float kTwoOverPi = 0.636619772367581343f;
+    const float P1 = 1.5703125f;                 // 8 significant bits
+    const float P2 = 4.83751296997070312500e-04f;
+    const float P3 = 7.54978995489188216e-08f;
+    const float kf = rintf(x * kTwoOverPi);
+    float r = fmaf(-kf, P1, x);
+    r = fmaf(-kf, P2, r);
+    r = fmaf(-kf, P3, r);
+    const float r2 = r * r;
+    // sin r = r (1 - r^2/3! + r^4/5! - r^6/7! + r^8/9! - r^10/11!)
+    float ps = -2.50521083854417187751e-08f;
+    ps = fmaf(ps, r2, 2.75573192239858906526e-06f);
+    ps = fmaf(ps, r2, -1.98412698412698412698e-04f);
+    ps = fmaf(ps, r2, 8.33333333333333333333e-03f);
+    ps = fmaf(ps, r2, -1.66666666666666666667e-01f);
+    const float sn = fmaf(ps * r2, r, r);
+    // cos r = 1 - r^2/2! + r^4/4! - r^6/6! + r^8/8! - r^10/10! + r^12/12!
+    float pc = 2.08767569878680989792e-09f;
+    pc = fmaf(pc, r2, -2.75573192239858906526e-07f);
+    pc = fmaf(pc, r2, 2.48015873015873015873e-05f);
+    pc = fmaf(pc, r2, -1.38888888888888888889e-03f);
+    pc = fmaf(pc, r2, 4.16666666666666666667e-02f);
+    pc = fmaf(pc, r2, -0.5f);
+    const float cs = fmaf(pc, r2, 1.0f);
+    const int q = (int)kf & 3;
+    float s = (q & 1) ? cs : sn;
+    float c = (q & 1) ? sn : cs;
+    if (q == 1 || q == 2) c = -c;
+    if (q & 2) s = -s;
+    *s_out = s;
+    *c_out = c;
 }
 
 }  // namespace emsim

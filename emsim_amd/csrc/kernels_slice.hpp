@@ -257,7 +257,7 @@ __global__ void k_add_water(float* hist, int n_elems, int64_t vox, int idx_h, in
 template <int N>
 __global__ __launch_bounds__(RowCfg<N>::BLOCK) void k_row_r2c_pairs(
     const float* __restrict__ in, float2* __restrict__ out, int64_t n_pairs, int rows_per_plane,
-    int planes_per_group, int64_t in_group_stride, int64_t in_plane0,
+    int planes_per_group, int64_t in_group_stride, int64_t in_plane0, int ldh,
     const float2* __restrict__ tw) {
     using Cfg = RowCfg<N>;
     constexpr int H = N / 2 + 1;
@@ -278,22 +278,22 @@ __global__ __launch_bounds__(RowCfg<N>::BLOCK) void k_row_r2c_pairs(
     const float* ra = in + (in_plane * rows_per_plane + 2 * q) * (int64_t)N;
     const float* rb = ra + N;
 
-    float2 v[kEPT];
+    float2 v[Cfg::EPT];
 #pragma unroll
-    for (int m = 0; m < kEPT; ++m)
+    for (int m = 0; m < Cfg::EPT; ++m)
         v[m] = valid ? make_float2(ra[t + m * Cfg::T], rb[t + m * Cfg::T]) : make_float2(0.f, 0.f);
-    fft_n<N, false>(v, re, im, t, tw);
+    fft_n<N, Cfg::EPT, false>(v, re, im, t, tw);
     // full spectrum Z to LDS, then split
 #pragma unroll
-    for (int m = 0; m < kEPT; ++m) {
+    for (int m = 0; m < Cfg::EPT; ++m) {
         const int idx = L::at(t + m * Cfg::T);
         re[idx] = v[m].x;
         im[idx] = v[m].y;
     }
     __syncthreads();
     if (valid) {
-        float2* oa = out + (p * rows_per_plane + 2 * q) * (int64_t)H;
-        float2* ob = oa + H;
+        float2* oa = out + (p * rows_per_plane + 2 * q) * (int64_t)ldh;
+        float2* ob = oa + ldh;
         for (int k = t; k < H; k += Cfg::T) {
             const int ik = L::at(k & (N - 1));
             const int inr = L::at((N - k) & (N - 1));
@@ -315,9 +315,8 @@ __global__ __launch_bounds__(RowCfg<N>::BLOCK) void k_row_r2c_pairs(
 template <int N>
 __global__ __launch_bounds__(RowCfg<N>::BLOCK) void k_row_c2r_pairs(
     const float2* __restrict__ in, float* __restrict__ out, int64_t n_pairs,
-    float clip_lo, const float2* __restrict__ tw) {
+    float clip_lo, int ldh, const float2* __restrict__ tw) {
     using Cfg = RowCfg<N>;
-    constexpr int H = N / 2 + 1;
     __shared__ float lds_re[Cfg::ROWS * Cfg::WORDS];
     __shared__ float lds_im[Cfg::ROWS * Cfg::WORDS];
     const int rib = threadIdx.x / Cfg::T;
@@ -327,11 +326,11 @@ __global__ __launch_bounds__(RowCfg<N>::BLOCK) void k_row_c2r_pairs(
     float* re = lds_re + rib * Cfg::WORDS;
     float* im = lds_im + rib * Cfg::WORDS;
 
-    const float2* ia = in + pair * 2 * (int64_t)H;
-    const float2* ib = ia + H;
-    float2 v[kEPT];
+    const float2* ia = in + pair * 2 * (int64_t)ldh;
+    const float2* ib = ia + ldh;
+    float2 v[Cfg::EPT];
 #pragma unroll
-    for (int m = 0; m < kEPT; ++m) {
+    for (int m = 0; m < Cfg::EPT; ++m) {
         const int k = t + m * Cfg::T;
         float2 z = make_float2(0.f, 0.f);
         if (valid) {
@@ -343,12 +342,12 @@ __global__ __launch_bounds__(RowCfg<N>::BLOCK) void k_row_c2r_pairs(
         }
         v[m] = z;
     }
-    fft_n<N, true>(v, re, im, t, tw);
+    fft_n<N, Cfg::EPT, true>(v, re, im, t, tw);
     if (valid) {
         float* oa = out + pair * 2 * (int64_t)N;
         float* ob = oa + N;
 #pragma unroll
-        for (int m = 0; m < kEPT; ++m) {
+        for (int m = 0; m < Cfg::EPT; ++m) {
             oa[t + m * Cfg::T] = fmaxf(v[m].x, clip_lo);
             ob[t + m * Cfg::T] = fmaxf(v[m].y, clip_lo);
         }
@@ -362,38 +361,52 @@ __global__ __launch_bounds__(RowCfg<N>::BLOCK) void k_row_c2r_pairs(
 // then * D(k1, j) / nPix and IFFT_col.  The result overwrites element 0's
 // plane (Y[sc] = X[0, sc]); every tile reads its element-0 columns before it
 // writes them, and tiles own disjoint columns.
-// grid.x = column tiles, grid.y = slices of the chunk.
+// grid.x = column tiles, grid.y = slices of the chunk.  X rows are ldh
+// complex apart (H padded to a multiple of 16, so every tile's segments are
+// 64-byte aligned); the form-factor table rows are H apart.
 // ---------------------------------------------------------------------------
 template <int N>
-__global__ __launch_bounds__(ColCfg<N>::BLOCK) void k_col_spectrum(
-    float2* X, const float* __restrict__ sf, int n_elems, int n_sc, int H, int n2,
+struct SpecCfg {
+    // 8 values per thread and 8 columns per tile: 512-thread blocks, room
+    // for the accumulator next to the transform without spilling
+    static constexpr int EPT = (N / 8) > 256 ? N / 256 : 8;
+    static constexpr int T = N / EPT;
+    static constexpr int C = (512 / T) < 1 ? 1 : ((512 / T) < 8 ? (512 / T) : 8);
+    static constexpr int BLOCK = T * C;
+    static constexpr int STRIDE = LdsLayout<N>::kStride;
+};
+
+template <int N>
+__global__ __launch_bounds__(SpecCfg<N>::BLOCK) void k_col_spectrum(
+    float2* X, const float* __restrict__ sf, int n_elems, int n_sc, int H, int ldh, int n2,
     int bandlimit, int band_r2, float inv_npix, const float2* __restrict__ tw) {
-    using Cfg = ColCfg<N>;
-    __shared__ float lds_re[Cfg::C * Cfg::WORDS];
-    __shared__ float lds_im[Cfg::C * Cfg::WORDS];
+    using Cfg = SpecCfg<N>;
+    __shared__ float lds_re[Cfg::C * Cfg::STRIDE];
+    __shared__ float lds_im[Cfg::C * Cfg::STRIDE];
     const int c = threadIdx.x % Cfg::C;
     const int t = threadIdx.x / Cfg::C;
     const int col = blockIdx.x * Cfg::C + c;
     const bool valid = col < H;
     const int sc = blockIdx.y;
-    float* re = lds_re + c * Cfg::WORDS;
-    float* im = lds_im + c * Cfg::WORDS;
-    const int64_t plane = (int64_t)N * H;
+    float* re = lds_re + c * Cfg::STRIDE;
+    float* im = lds_im + c * Cfg::STRIDE;
+    const int64_t plane = (int64_t)N * ldh;
+    const int64_t sf_plane = (int64_t)N * H;
 
-    float2 acc[kEPT];
+    float2 acc[Cfg::EPT];
 #pragma unroll
-    for (int m = 0; m < kEPT; ++m) acc[m] = make_float2(0.f, 0.f);
+    for (int m = 0; m < Cfg::EPT; ++m) acc[m] = make_float2(0.f, 0.f);
 
     for (int e = 0; e < n_elems; ++e) {
         const float2* src = X + ((int64_t)e * n_sc + sc) * plane + col;
-        float2 v[kEPT];
+        float2 v[Cfg::EPT];
 #pragma unroll
-        for (int m = 0; m < kEPT; ++m)
-            v[m] = valid ? src[(int64_t)(t + m * Cfg::T) * H] : make_float2(0.f, 0.f);
-        fft_n<N, false>(v, re, im, t, tw);
-        const float* f = sf + (int64_t)e * plane + col;
+        for (int m = 0; m < Cfg::EPT; ++m)
+            v[m] = valid ? src[(int64_t)(t + m * Cfg::T) * ldh] : make_float2(0.f, 0.f);
+        fft_n<N, Cfg::EPT, false>(v, re, im, t, tw);
+        const float* f = sf + (int64_t)e * sf_plane + col;
 #pragma unroll
-        for (int m = 0; m < kEPT; ++m) {
+        for (int m = 0; m < Cfg::EPT; ++m) {
             const float fe = valid ? f[(int64_t)(t + m * Cfg::T) * H] : 0.f;
             acc[m].x += v[m].x * fe;
             acc[m].y += v[m].y * fe;
@@ -403,7 +416,7 @@ __global__ __launch_bounds__(ColCfg<N>::BLOCK) void k_col_spectrum(
     // (slice_builder_base.py:166-175), centred as ifftshift(arange(-n//2, ...)).
     const int kj = freq_int(col, n2);
 #pragma unroll
-    for (int m = 0; m < kEPT; ++m) {
+    for (int m = 0; m < Cfg::EPT; ++m) {
         const int k1 = t + m * Cfg::T;
         float s = inv_npix;
         if (bandlimit) {
@@ -412,11 +425,11 @@ __global__ __launch_bounds__(ColCfg<N>::BLOCK) void k_col_spectrum(
         }
         acc[m] = make_float2(acc[m].x * s, acc[m].y * s);
     }
-    fft_n<N, true>(acc, re, im, t, tw);
+    fft_n<N, Cfg::EPT, true>(acc, re, im, t, tw);
     if (valid) {
         float2* dst = X + (int64_t)sc * plane + col;
 #pragma unroll
-        for (int m = 0; m < kEPT; ++m) dst[(int64_t)(t + m * Cfg::T) * H] = acc[m];
+        for (int m = 0; m < Cfg::EPT; ++m) dst[(int64_t)(t + m * Cfg::T) * ldh] = acc[m];
     }
 }
 

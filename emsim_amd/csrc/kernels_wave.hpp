@@ -37,36 +37,65 @@ __device__ __forceinline__ float freq2(int i, int j, int n1, int n2, float dfx, 
 }
 
 // Fresnel propagator with the 2/3 mask and the 1/nPix of the unnormalised
-// inverse (WavePropagator.cpp:62-85).
+// inverse (WavePropagator.cpp:62-85).  The kept set is decided exactly as
+// the reference does (float32 f^2 <= filter^2); the phase factor
+// exp(-i lpd f^2) is applied as exp(-i lpd fx^2) exp(-i lpd fy^2), whose
+// factors each block tabulates once in LDS (prepare), so no element needs a
+// sin/cos.  The split changes the float32 phase by < 1e-8 rad.
 struct PropagatorMul {
     int n1, n2;
     float dfx, dfy;
     float filter2;    // (0.6667 fmax)^2
     float lpd;        // (lambda * pi) * dz
     float inv_npix;   // 1/nPix, exact for power-of-two sizes
-    __device__ __forceinline__ float2 operator()(float2 w, int i, int j) const {
+    static constexpr bool kTable = true;
+
+    // px[i] = exp(-i lpd fx_i^2) for all n1 rows (called by every thread of
+    // the block, followed by a barrier in the caller).
+    __device__ __forceinline__ void prepare(float2* px, int tid, int nthreads) const {
+        for (int i = tid; i < n1; i += nthreads) {
+            const float fx = __fmul_rn((float)freq_int(i, n1), dfx);
+            float s, c;
+            fast_sincos(__fmul_rn(lpd, __fmul_rn(fx, fx)), &s, &c);
+            px[i] = make_float2(c, -s);
+        }
+    }
+    // column factor exp(-i lpd fy_j^2) / nPix
+    __device__ __forceinline__ float2 column(int j) const {
+        const float fy = __fmul_rn((float)freq_int(j, n2), dfy);
+        float s, c;
+        fast_sincos(__fmul_rn(lpd, __fmul_rn(fy, fy)), &s, &c);
+        return make_float2(c * inv_npix, -s * inv_npix);
+    }
+    __device__ __forceinline__ float2 apply(float2 w, int i, int j, const float2* px,
+                                            float2 py) const {
         const float f2 = freq2(i, j, n1, n2, dfx, dfy);
         if (!(f2 <= filter2)) return make_float2(0.f, 0.f);
-        float s, c;
-        sincosf(__fmul_rn(lpd, f2), &s, &c);
-        // (w.x + i w.y) (c - i s)
-        return make_float2((w.x * c + w.y * s) * inv_npix, (w.y * c - w.x * s) * inv_npix);
+        return cmul(w, cmul(px[i], py));
     }
 };
 
 // Lens transfer function exp(-i chi) with the aperture (WavePropagator.cpp:105-135).
+// chi = c1 f^4 - c2 f^2 does not factor over the axes, so each kept element
+// evaluates its own sin/cos (one launch per image).
 struct LensMul {
     int n1, n2;
     float dfx, dfy;
     float faper2;     // (aperture / lambda)^2
     float c1, c2;     // chi = c1 f^4 - c2 f^2
     float inv_npix;
+    static constexpr bool kTable = false;
+    __device__ __forceinline__ void prepare(float2*, int, int) const {}
+    __device__ __forceinline__ float2 column(int) const { return make_float2(1.f, 0.f); }
+    __device__ __forceinline__ float2 apply(float2 w, int i, int j, const float2*, float2) const {
+        return (*this)(w, i, j);
+    }
     __device__ __forceinline__ float2 operator()(float2 w, int i, int j) const {
         const float f2 = freq2(i, j, n1, n2, dfx, dfy);
         if (!(f2 <= faper2)) return make_float2(0.f, 0.f);
         const float chi = __fsub_rn(__fmul_rn(__fmul_rn(c1, f2), f2), __fmul_rn(c2, f2));
         float s, c;
-        sincosf(chi, &s, &c);
+        fast_sincos(chi, &s, &c);
         return make_float2((w.x * c + w.y * s) * inv_npix, (w.y * c - w.x * s) * inv_npix);
     }
 };
@@ -76,30 +105,54 @@ struct LensMul {
 // ---------------------------------------------------------------------------
 template <int N>
 struct RowCfg {
-    static constexpr int T = N / kEPT;                       // threads per row
+    static constexpr int EPT = 8;
+    static constexpr int T = N / EPT;                        // threads per row
     static constexpr int ROWS = T >= 256 ? 1 : 256 / T;      // rows per block
     static constexpr int BLOCK = T * ROWS;
     static constexpr int WORDS = LdsLayout<N>::kWords;       // floats per row plane
 };
 
+// Column tiles: C adjacent columns per block (16 -> 128-byte contiguous
+// segments per access), EPT values per thread, at most kColMaxBlock threads.
+#ifndef EMSIM_COL_EPT
+#define EMSIM_COL_EPT 8
+#endif
+#ifndef EMSIM_COL_MAX_BLOCK
+#define EMSIM_COL_MAX_BLOCK 1024
+#endif
 template <int N>
 struct ColCfg {
-    static constexpr int T = N / kEPT;                       // threads per column
-    // columns per tile; blocks stay at <= 512 threads so that a thread may use
-    // up to 256 VGPRs without spilling
-    static constexpr int C = (512 / T) < 1 ? 1 : ((512 / T) < 16 ? (512 / T) : 16);
+    static constexpr int EPT = (N / EMSIM_COL_EPT) > 256 ? N / 256
+                               : (EMSIM_COL_EPT < N ? EMSIM_COL_EPT : 8);
+    static constexpr int T = N / EPT;                        // threads per column
+    static constexpr int STRIDE = LdsLayout<N>::kStride;     // odd: columns on distinct banks
+    // threads per block, and 96 KiB of exchange planes (the rest of the
+    // 160 KiB LDS holds the propagator's row table)
+    static constexpr int CMAX_T = EMSIM_COL_MAX_BLOCK / T;
+    static constexpr int CMAX_L = (96 * 1024) / (8 * STRIDE);
+    static constexpr int CMAX = CMAX_T < CMAX_L ? CMAX_T : CMAX_L;
+    static constexpr int C = CMAX < 1 ? 1 : (CMAX < 16 ? CMAX : 16);
     static constexpr int BLOCK = T * C;
-    static constexpr int WORDS = LdsLayout<N>::kWords;
 };
+
+// Row index of this thread's transform; wave-uniform when a row spans whole
+// waves, so it lives in a scalar register.
+template <int T>
+__device__ __forceinline__ int row_in_block() {
+    const int r = threadIdx.x / T;
+    if constexpr (T % 64 == 0) return __builtin_amdgcn_readfirstlane(r);
+    else return r;
+}
 
 // ---------------------------------------------------------------------------
 // row kernels
 // ---------------------------------------------------------------------------
 #define EMSIM_ROW_PROLOGUE(N)                                                   \
     using Cfg = RowCfg<N>;                                                      \
+    constexpr int EPT = Cfg::EPT;                                               \
     __shared__ float lds_re[Cfg::ROWS * Cfg::WORDS];                            \
     __shared__ float lds_im[Cfg::ROWS * Cfg::WORDS];                            \
-    const int rib = threadIdx.x / Cfg::T;                                       \
+    const int rib = row_in_block<Cfg::T>();                                     \
     const int t = threadIdx.x % Cfg::T;                                         \
     const int64_t row = (int64_t)blockIdx.x * Cfg::ROWS + rib;                  \
     const bool valid = row < n_rows;                                            \
@@ -112,15 +165,15 @@ __global__ __launch_bounds__(RowCfg<N>::BLOCK) void k_row_fft(const float2* in, 
                                                                int64_t n_rows,
                                                                const float2* __restrict__ tw) {
     EMSIM_ROW_PROLOGUE(N)
-    float2 v[kEPT];
+    float2 v[EPT];
     const float2* src = in + row * N;
 #pragma unroll
-    for (int m = 0; m < kEPT; ++m) v[m] = valid ? src[t + m * Cfg::T] : make_float2(0.f, 0.f);
-    fft_n<N, INV>(v, re, im, t, tw);
+    for (int m = 0; m < EPT; ++m) v[m] = valid ? src[t + m * Cfg::T] : make_float2(0.f, 0.f);
+    fft_n<N, EPT, INV>(v, re, im, t, tw);
     if (valid) {
         float2* dst = out + row * N;
 #pragma unroll
-        for (int m = 0; m < kEPT; ++m) dst[t + m * Cfg::T] = v[m];
+        for (int m = 0; m < EPT; ++m) dst[t + m * Cfg::T] = v[m];
     }
 }
 
@@ -133,27 +186,27 @@ __global__ __launch_bounds__(RowCfg<N>::BLOCK) void k_row_transmit_fft(
     const float2* in, const float* __restrict__ V, float2* out, int64_t n_rows,
     int rows_per_img, int64_t v_img_stride, float factor, const float2* __restrict__ tw) {
     EMSIM_ROW_PROLOGUE(N)
-    float2 v[kEPT];
+    float2 v[EPT];
     const float2* src = in + row * N;
 #pragma unroll
-    for (int m = 0; m < kEPT; ++m) v[m] = valid ? src[t + m * Cfg::T] : make_float2(0.f, 0.f);
-    if constexpr (!FIRST) fft_n<N, true>(v, re, im, t, tw);
+    for (int m = 0; m < EPT; ++m) v[m] = valid ? src[t + m * Cfg::T] : make_float2(0.f, 0.f);
+    if constexpr (!FIRST) fft_n<N, EPT, true>(v, re, im, t, tw);
     if (valid) {
         const int64_t img = row / rows_per_img;
         const float* vrow = V + img * v_img_stride + (row - img * rows_per_img) * N;
 #pragma unroll
-        for (int m = 0; m < kEPT; ++m) {
+        for (int m = 0; m < EPT; ++m) {
             float s, c;
-            sincosf(__fmul_rn(vrow[t + m * Cfg::T], factor), &s, &c);
+            fast_sincos(__fmul_rn(vrow[t + m * Cfg::T], factor), &s, &c);
             const float2 w = v[m];
             v[m] = make_float2(w.x * c - w.y * s, w.x * s + w.y * c);
         }
     }
-    fft_n<N, false>(v, re, im, t, tw);
+    fft_n<N, EPT, false>(v, re, im, t, tw);
     if (valid) {
         float2* dst = out + row * N;
 #pragma unroll
-        for (int m = 0; m < kEPT; ++m) dst[t + m * Cfg::T] = v[m];
+        for (int m = 0; m < EPT; ++m) dst[t + m * Cfg::T] = v[m];
     }
 }
 
@@ -163,15 +216,15 @@ template <int N>
 __global__ __launch_bounds__(RowCfg<N>::BLOCK) void k_row_ifft_intensity(
     const float2* in, float* __restrict__ img_out, int64_t n_rows, const float2* __restrict__ tw) {
     EMSIM_ROW_PROLOGUE(N)
-    float2 v[kEPT];
+    float2 v[EPT];
     const float2* src = in + row * N;
 #pragma unroll
-    for (int m = 0; m < kEPT; ++m) v[m] = valid ? src[t + m * Cfg::T] : make_float2(0.f, 0.f);
-    fft_n<N, true>(v, re, im, t, tw);
+    for (int m = 0; m < EPT; ++m) v[m] = valid ? src[t + m * Cfg::T] : make_float2(0.f, 0.f);
+    fft_n<N, EPT, true>(v, re, im, t, tw);
     if (valid) {
         float* dst = img_out + row * N;
 #pragma unroll
-        for (int m = 0; m < kEPT; ++m) dst[t + m * Cfg::T] = v[m].x * v[m].x + v[m].y * v[m].y;
+        for (int m = 0; m < EPT; ++m) dst[t + m * Cfg::T] = v[m].x * v[m].x + v[m].y * v[m].y;
     }
 }
 
@@ -185,27 +238,32 @@ template <int N, class Mul>
 __global__ __launch_bounds__(ColCfg<N>::BLOCK) void k_col_fft_mul_ifft(
     float2* data, int ld, int ncols, int64_t img_stride, Mul mul, const float2* __restrict__ tw) {
     using Cfg = ColCfg<N>;
-    __shared__ float lds_re[Cfg::C * Cfg::WORDS];
-    __shared__ float lds_im[Cfg::C * Cfg::WORDS];
+    constexpr int EPT = Cfg::EPT;
+    __shared__ float lds_re[Cfg::C * Cfg::STRIDE];
+    __shared__ float lds_im[Cfg::C * Cfg::STRIDE];
+    __shared__ float2 lds_px[Mul::kTable ? N : 1];
     const int c = threadIdx.x % Cfg::C;
     const int t = threadIdx.x / Cfg::C;
     const int col = blockIdx.x * Cfg::C + c;
     const bool valid = col < ncols;
     float2* base = data + (int64_t)blockIdx.y * img_stride + col;
-    float* re = lds_re + c * Cfg::WORDS;
-    float* im = lds_im + c * Cfg::WORDS;
+    float* re = lds_re + c * Cfg::STRIDE;
+    float* im = lds_im + c * Cfg::STRIDE;
 
-    float2 v[kEPT];
+    float2 v[EPT];
 #pragma unroll
-    for (int m = 0; m < kEPT; ++m)
+    for (int m = 0; m < EPT; ++m)
         v[m] = valid ? base[(int64_t)(t + m * Cfg::T) * ld] : make_float2(0.f, 0.f);
-    fft_n<N, false>(v, re, im, t, tw);
+    mul.prepare(lds_px, threadIdx.x, Cfg::BLOCK);
+    if constexpr (Mul::kTable) __syncthreads();
+    const float2 py = mul.column(col);
+    fft_n<N, EPT, false>(v, re, im, t, tw);
 #pragma unroll
-    for (int m = 0; m < kEPT; ++m) v[m] = mul(v[m], t + m * Cfg::T, col);
-    fft_n<N, true>(v, re, im, t, tw);
+    for (int m = 0; m < EPT; ++m) v[m] = mul.apply(v[m], t + m * Cfg::T, col, lds_px, py);
+    fft_n<N, EPT, true>(v, re, im, t, tw);
     if (valid) {
 #pragma unroll
-        for (int m = 0; m < kEPT; ++m) base[(int64_t)(t + m * Cfg::T) * ld] = v[m];
+        for (int m = 0; m < EPT; ++m) base[(int64_t)(t + m * Cfg::T) * ld] = v[m];
     }
 }
 
@@ -232,7 +290,7 @@ __global__ void k_transmit(const float2* in, const float* __restrict__ V, float2
         const int64_t img = i / npix;
         const float pix = V[img * v_img_stride + (i - img * npix)];
         float s, c;
-        sincosf(__fmul_rn(pix, factor), &s, &c);
+        fast_sincos(__fmul_rn(pix, factor), &s, &c);
         const float2 w = in[i];
         out[i] = make_float2(w.x * c - w.y * s, w.x * s + w.y * c);
     }
