@@ -36,6 +36,8 @@ def parse():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--slices", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--unfused", action="store_true",
+                    help="time the reference-shaped kernels (histogram, make_slices, multislice)")
     ap.add_argument("--no-timing", action="store_true",
                     help="do not bracket launches with HIP events")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -74,21 +76,26 @@ def workload(batch: int, n_atoms_range=(10000, 100000), n_base: int = 8, seed: i
 def kernel_bytes_per_launch(name: str, n: int, cfg) -> float:
     """Algorithmic HBM bytes of one launch of a kernel in this workload.
 
-    Per real pixel per slice (h = (n/2+1)/n, nE elements):
-      k_row_transmit_fft   pass A: wave in 8 + V 4 + wave out 8      = 20
-      k_col_fft_mul_ifft   pass B: wave in 8 + out 8                 = 16
-      k_row_r2c_pairs      hist in 4 nE + half spectra out 8h nE
-      k_col_spectrum       half spectra in 8h nE + out 8h (+ table 4h nE once)
-      k_row_c2r_pairs      half spectrum in 8h + V out 4
-      k_bin_tiles          histogram out 4 nE
-      k_row_fft            wave in 8 + out 8
-      k_row_ifft_intensity wave in 8 + image out 4
-    One pass-A/B launch covers the batch; one potential-build launch covers
-    the slices of one molecule.
+    Per real pixel per slice (h = (n/2+1)/n, nE elements), the reference's
+    passes are charged at:
+      binning              histogram out 4 nE
+      R2C                  histogram in 4 nE + half spectra out 8h nE
+      spectrum column pass half spectra in 8h nE + out 8h
+      C2R + clip           half spectrum in 8h + V out 4
+      pass A               wave in 8 + V 4 + wave out 8 = 20
+      pass B               wave in 8 + out 8 = 16
+      lens epilogue        wave in 8 + image out 4 = 12
+    The fused kernels are charged for the passes they replace:
+      k_spec_sparse        binning + R2C + spectrum pass = nE (8 + 16h) + 8h
+      k_slice_pass_a       C2R + pass A = 8h + 24
+    Fused kernels and pass A/B cover one slice of the whole batch per launch;
+    the unfused potential-build kernels cover all slices of one molecule.
     """
     B, npix, nS, nE = cfg["batch"], cfg["n"] ** 2, cfg["slices"], cfg["n_elems"]
     h = (cfg["n"] // 2 + 1) / cfg["n"]
     per = {
+        "k_spec_sparse": (nE * (8 + 16 * h) + 8 * h) * npix * B,
+        "k_slice_pass_a": (8 * h + 24.0) * npix * B,
         "k_row_transmit_fft": 20.0 * npix * B,
         "k_col_fft_mul_ifft": 16.0 * npix * B,
         "k_row_fft": 16.0 * npix * B,
@@ -150,11 +157,12 @@ def main():
                             aperture=10e-3)
     mols, sizes = workload(args.batch)
     pipe = BatchPipe(tem, cfg["pixel"], cfg["dz"], cfg["n"], cfg["slices"], cfg["elements"])
-    prepared = [pipe.prepare(m) for m in mols]
+    prepared = pipe.upload([pipe.prepare(m) for m in mols])
+    run = pipe.run_unfused if args.unfused else pipe.run
     torch.cuda.synchronize()
 
     for _ in range(args.warmup):
-        pipe.run(prepared)
+        run(prepared)
     torch.cuda.synchronize()
 
     if not args.no_timing:
@@ -163,7 +171,7 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        images = pipe.run(prepared)
+        images = run(prepared)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     kernels = []

@@ -62,6 +62,10 @@ _SIGNATURES = {
     "emsim_hip_lens_intensity": (c_int, [c_void_p, c_void_p, c_float, c_float, c_float,
                                          c_void_p, c_void_p, c_int, c_void_p]),
     "emsim_hip_intensity": (c_int, [c_void_p, c_void_p, c_int64, c_void_p]),
+    "emsim_hip_simulate_workspace_size": (c_size_t, [c_void_p, c_int, c_int64, c_int]),
+    "emsim_hip_simulate": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, POINTER(c_int64),
+                                   POINTER(c_int64), c_int, c_float, c_float, c_float, c_float,
+                                   c_int, c_void_p, c_int, c_void_p, c_size_t, c_void_p]),
     "emsim_hip_timing_enable": (c_int, [c_int]),
     "emsim_hip_timing_collect": (c_int, [c_void_p, c_size_t, POINTER(c_size_t)]),
 }

@@ -5,15 +5,18 @@ calls ``emsim/pipe.py:79-84`` per molecule, and ``Pipe`` rebuilds its
 propagator three times per image, ``pipe.py:31-34``).  ``BatchPipe`` runs
 the same per-image algorithm -- centralize, bin, build slices, multislice,
 lens, |psi|^2 -- for a batch of molecules whose atoms are already in GPU
-memory, and propagates all waves of the batch together so every kernel
-launch covers the whole batch.  Unlike ``Pipe`` it passes ``n_slices`` to
-the slice builder, so every image has the same slice count.
+memory.  Unlike ``Pipe`` it passes ``n_slices`` to the slice builder, so
+every image has the same slice count.
 
-Steps per batch (all on the current HIP stream):
-  for each molecule: bin atoms -> (nE, nS, n1, n2) histogram (reused buffer)
-                     build slices -> V[b] (nS, n1, n2)
-  init B plane waves; multislice the batch through V; lens + |psi|^2.
+``run`` / ``exit_waves`` use the fused kernels (``emsim_hip_simulate``):
+atoms are bucketed once per batch, and each slice costs three launches for
+the whole batch (spectrum from the atom buckets, potential C2R fused into the
+transmission row pass, propagator column pass), with the lens folded into the
+last slice.  ``run_unfused`` is the same computation through the
+reference-shaped steps (dense histogram, ``make_multi_slices``,
+``multislice_propagate``, ``lens_intensity``), kept as a cross-check.
 """
+from ctypes import POINTER, c_int64
 from dataclasses import dataclass
 from typing import List, Sequence
 
@@ -21,7 +24,8 @@ import numpy as np
 import torch
 
 from . import atoms as atm
-from .backend._device import as_device, device
+from .backend import hip_ext
+from .backend._device import as_device, device, ptr, stream_handle
 from .backend.slice_builder_hip import MultiSlicesBuilder
 from .backend.wave_propagator_hip import WavePropagator
 from .tem import TEM
@@ -29,9 +33,20 @@ from .tem import TEM
 
 @dataclass
 class PreparedMolecule:
-    coords: torch.Tensor      # (n_atoms, 3) float32 on the GPU, sorted by element
+    coords: np.ndarray        # (n_atoms, 3) float32, centred, sorted by element
     counts: np.ndarray        # (nE,) int64 atoms per element of the pipe's element list
     label: str = ""
+
+
+@dataclass
+class PreparedBatch:
+    coords: torch.Tensor      # (total atoms, 3) float32 on the GPU
+    mol_off: np.ndarray       # (B+1,) int64
+    counts: np.ndarray        # (B, nE) int64
+    labels: List[str]
+
+    def __len__(self):
+        return len(self.labels)
 
 
 class BatchPipe:
@@ -53,15 +68,28 @@ class BatchPipe:
 
     # -- host preparation (outside any timed region) -----------------------
     def prepare(self, mol: atm.AtomList) -> PreparedMolecule:
-        """Centre, sort by element and upload one molecule."""
+        """Centre and sort one molecule by element (host side)."""
         mol = atm.centralize(mol)
         unknown = set(np.unique(mol.elements).tolist()) - set(self.elements)
         if unknown:
             raise ValueError(f"molecule has elements {sorted(unknown)} outside the pipe's "
                              f"element list {self.elements}")
         mol, zs, counts = atm.sort_elements_and_count(mol, self.elements)
-        return PreparedMolecule(as_device(mol.coordinates, torch.float32, "coordinates"),
+        return PreparedMolecule(np.ascontiguousarray(mol.coordinates, dtype=np.float32),
                                 np.asarray(counts, dtype=np.int64), mol.label)
+
+    def upload(self, mols: Sequence[PreparedMolecule]) -> PreparedBatch:
+        """Concatenate prepared molecules and copy their atoms to the GPU once."""
+        if len(mols) == 0:
+            raise ValueError("empty batch")
+        sizes = np.array([m.coords.shape[0] for m in mols], dtype=np.int64)
+        mol_off = np.zeros(len(mols) + 1, dtype=np.int64)
+        np.cumsum(sizes, out=mol_off[1:])
+        coords = np.concatenate([m.coords for m in mols]) if mol_off[-1] else \
+            np.zeros((0, 3), np.float32)
+        counts = np.stack([m.counts for m in mols]).astype(np.int64)
+        return PreparedBatch(as_device(coords, torch.float32, "coordinates"), mol_off,
+                             np.ascontiguousarray(counts), [m.label for m in mols])
 
     def _buffer(self, name, shape, dtype):
         t = self._bufs.get(name)
@@ -70,25 +98,51 @@ class BatchPipe:
             self._bufs[name] = t
         return t
 
-    # -- GPU work ------------------------------------------------------------
-    def build_slices(self, mols: List[PreparedMolecule]) -> torch.Tensor:
+    # -- fused GPU path --------------------------------------------------------
+    def _simulate(self, batch: PreparedBatch, mode: int) -> torch.Tensor:
+        B = len(batch)
+        n_atoms = batch.coords.shape[0]
+        if mode == 0:
+            out = torch.empty((B, self.n, self.n), dtype=torch.float32, device=device())
+        else:
+            out = torch.empty((B, self.n, self.n), dtype=torch.complex64, device=device())
+        ws_bytes = hip_ext.raw("emsim_hip_simulate_workspace_size")(
+            self.builder.backend.h, B, n_atoms, mode)
+        ws = self._buffer("simulate_ws", (max(ws_bytes, 1),), torch.uint8)
+        mic = self.microscope
+        psi0 = self.propagator.init_wave_value(mic.electron_dose)
+        hip_ext.call("emsim_hip_simulate", self.builder.backend.h, self.propagator.backend.h,
+                     ptr(batch.coords), n_atoms, batch.mol_off.ctypes.data_as(POINTER(c_int64)),
+                     batch.counts.ctypes.data_as(POINTER(c_int64)), B, float(psi0),
+                     float(mic.cs_mm), float(mic.defocus), float(mic.aperture),
+                     1 if self.symmetric_bandlimit else 0, ptr(out), mode, ptr(ws), ws.numel(),
+                     stream_handle())
+        return out
+
+    def run(self, batch: PreparedBatch) -> torch.Tensor:
+        """(B, n, n) float32 images on the GPU."""
+        return self._simulate(batch, 0)
+
+    def exit_waves(self, batch: PreparedBatch) -> torch.Tensor:
+        """(B, n, n) complex64 exit waves on the GPU."""
+        return self._simulate(batch, 1)
+
+    # -- reference-shaped GPU path (cross-check) --------------------------------
+    def build_slices(self, batch: PreparedBatch) -> torch.Tensor:
         nE, nS, n = len(self.elements), self.n_slices, self.n
-        V = self._buffer("V", (len(mols), nS, n, n), torch.float32)
+        V = self._buffer("V", (len(batch), nS, n, n), torch.float32)
         hist = self._buffer("hist", (nE, nS, n, n), torch.float32)
         ws = self._buffer("ws", (self.builder.backend.slices_workspace_bytes(),), torch.uint8)
-        for b, m in enumerate(mols):
-            self.builder.backend.bin_atoms(m.coords, m.counts, out=hist)
+        for b in range(len(batch)):
+            lo, hi = batch.mol_off[b], batch.mol_off[b + 1]
+            self.builder.backend.bin_atoms(batch.coords[lo:hi], batch.counts[b], out=hist)
             self.builder.backend.make_slices(hist, self.symmetric_bandlimit, out=V[b], ws=ws)
         return V
 
-    def exit_waves(self, mols: List[PreparedMolecule]) -> torch.Tensor:
-        V = self.build_slices(mols)
-        w0 = self.propagator.init_wave(self.microscope.electron_dose, batch=len(mols))
-        return self.propagator.multislice_propagate(w0, V, self.dz)
-
-    def run(self, mols: List[PreparedMolecule]) -> torch.Tensor:
-        """(B, n, n) float32 images on the GPU."""
-        waves = self.exit_waves(mols)
-        return self.propagator.lens_intensity(waves, self.microscope.cs_mm,
-                                              self.microscope.defocus, self.microscope.aperture,
+    def run_unfused(self, batch: PreparedBatch) -> torch.Tensor:
+        V = self.build_slices(batch)
+        w0 = self.propagator.init_wave(self.microscope.electron_dose, batch=len(batch))
+        waves = self.propagator.multislice_propagate(w0, V, self.dz)
+        mic = self.microscope
+        return self.propagator.lens_intensity(waves, mic.cs_mm, mic.defocus, mic.aperture,
                                               overwrite_wave=True)

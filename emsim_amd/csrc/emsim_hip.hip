@@ -18,6 +18,7 @@
 #include "fft_lds.hpp"
 #include "kernels_slice.hpp"
 #include "kernels_wave.hpp"
+#include "kernels_fused.hpp"
 
 using namespace emsim;
 
@@ -112,6 +113,35 @@ bool pow2_supported(int n) { return n >= 8 && n <= kTwN && (n & (n - 1)) == 0; }
 // --------------------------------------------------------------------------
 std::mutex g_tw_mutex;
 std::vector<float2*> g_twiddles;
+std::vector<int2*> g_twiddles_q30;
+
+// The same table in Q30 fixed point, round(2^30 cos), round(2^30 sin), for
+// the order-independent sparse DFT sums of the fused path.
+int get_twiddles_q30(const int2** out) {
+    int dev = 0;
+    EMSIM_HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_tw_mutex);
+    if ((int)g_twiddles_q30.size() <= dev) g_twiddles_q30.resize(dev + 1, nullptr);
+    if (!g_twiddles_q30[dev]) {
+        std::vector<int2> host(kTwN);
+        const double two_pi = 6.283185307179586476925286766559;
+        for (int m = 0; m < kTwN; ++m) {
+            const double a = -two_pi * (double)m / (double)kTwN;
+            host[m] = make_int2((int)std::llround(std::cos(a) * 1073741824.0),
+                                (int)std::llround(std::sin(a) * 1073741824.0));
+        }
+        int2* d = nullptr;
+        EMSIM_HIP_CHECK(hipMalloc(&d, kTwN * sizeof(int2)));
+        hipError_t err = hipMemcpy(d, host.data(), kTwN * sizeof(int2), hipMemcpyHostToDevice);
+        if (err != hipSuccess) {
+            (void)hipFree(d);
+            return fail(EMSIM_HIP_EDEVICE, "twiddle upload failed: %s", hipGetErrorString(err));
+        }
+        g_twiddles_q30[dev] = d;
+    }
+    *out = g_twiddles_q30[dev];
+    return EMSIM_HIP_OK;
+}
 
 int get_twiddles(const float2** out) {
     int dev = 0;
@@ -228,13 +258,24 @@ struct emsim_hip_wave_propagator {
 
 namespace {
 
+// Grid of the row worker for n_items rows (or row pairs) of length N.
+template <int N>
+void row_grid(int64_t n_items, int& rpw, dim3& grid) {
+    using Cfg = RowCfg<N>;
+    const int64_t groups = ceil_div(n_items, Cfg::G);
+    rpw = rows_per_block(groups);
+    grid = dim3((unsigned)ceil_div(groups, rpw));
+}
+
 template <int N>
 int launch_row_fft(bool inv, const float2* in, float2* out, int64_t rows, const float2* tw,
                    hipStream_t s) {
-    using Cfg = RowCfg<N>;
-    const dim3 grid((unsigned)ceil_div(rows, Cfg::ROWS));
-    if (inv) EMSIM_LAUNCH(N, (k_row_fft<N, true>), grid, dim3(Cfg::BLOCK), 0, s, in, out, rows, tw);
-    else EMSIM_LAUNCH(N, (k_row_fft<N, false>), grid, dim3(Cfg::BLOCK), 0, s, in, out, rows, tw);
+    int rpw;
+    dim3 grid;
+    row_grid<N>(rows, rpw, grid);
+    const dim3 block(RowCfg<N>::BLOCK);
+    if (inv) EMSIM_LAUNCH(N, (k_row_fft<N, true>), grid, block, 0, s, OpFFT<N, true>{in, out}, rows, rpw, tw);
+    else EMSIM_LAUNCH(N, (k_row_fft<N, false>), grid, block, 0, s, OpFFT<N, false>{in, out}, rows, rpw, tw);
     return EMSIM_HIP_OK;
 }
 
@@ -242,23 +283,29 @@ template <int N>
 int launch_row_transmit_fft(bool first, const float2* in, const float* V, float2* out,
                             int64_t rows, int rows_per_img, int64_t v_img_stride, float factor,
                             const float2* tw, hipStream_t s) {
-    using Cfg = RowCfg<N>;
-    const dim3 grid((unsigned)ceil_div(rows, Cfg::ROWS));
+    int rpw;
+    dim3 grid;
+    row_grid<N>(rows, rpw, grid);
+    const dim3 block(RowCfg<N>::BLOCK);
     if (first)
-        EMSIM_LAUNCH(N, (k_row_transmit_fft<N, true>), grid, dim3(Cfg::BLOCK), 0, s, in, V,
-                           out, rows, rows_per_img, v_img_stride, factor, tw);
+        EMSIM_LAUNCH(N, (k_row_transmit_fft<N, true>), grid, block, 0, s,
+                     (OpTransmit<N, true>{in, V, out, rows_per_img, v_img_stride, factor}), rows,
+                     rpw, tw);
     else
-        EMSIM_LAUNCH(N, (k_row_transmit_fft<N, false>), grid, dim3(Cfg::BLOCK), 0, s, in, V,
-                           out, rows, rows_per_img, v_img_stride, factor, tw);
+        EMSIM_LAUNCH(N, (k_row_transmit_fft<N, false>), grid, block, 0, s,
+                     (OpTransmit<N, false>{in, V, out, rows_per_img, v_img_stride, factor}), rows,
+                     rpw, tw);
     return EMSIM_HIP_OK;
 }
 
 template <int N>
 int launch_row_ifft_intensity(const float2* in, float* out, int64_t rows, const float2* tw,
                               hipStream_t s) {
-    using Cfg = RowCfg<N>;
-    EMSIM_LAUNCH(N, (k_row_ifft_intensity<N>), dim3((unsigned)ceil_div(rows, Cfg::ROWS)),
-                       dim3(Cfg::BLOCK), 0, s, in, out, rows, tw);
+    int rpw;
+    dim3 grid;
+    row_grid<N>(rows, rpw, grid);
+    EMSIM_LAUNCH(N, (k_row_ifft_intensity<N>), grid, dim3(RowCfg<N>::BLOCK), 0, s,
+                 (OpIfftIntensity<N>{in, out}), rows, rpw, tw);
     return EMSIM_HIP_OK;
 }
 
@@ -266,9 +313,13 @@ template <int N, class Mul>
 int launch_col_mul_impl(float2* data, int ld, int ncols, int64_t img_stride, int n_img, Mul mul,
                         const float2* tw, hipStream_t s) {
     using Cfg = ColCfg<N>;
-    const dim3 grid((unsigned)ceil_div(ncols, Cfg::C), (unsigned)n_img);
-    EMSIM_LAUNCH(N, (k_col_fft_mul_ifft<N, Mul>), grid, dim3(Cfg::BLOCK), 0, s, data, ld,
-                       ncols, img_stride, mul, tw);
+    const int tiles_per_img = (int)ceil_div(ncols, Cfg::C);
+    const int64_t n_tiles = (int64_t)tiles_per_img * n_img;
+    int64_t tpb = n_tiles / 512;
+    tpb = tpb < 1 ? 1 : (tpb > 8 ? 8 : tpb);
+    const dim3 grid((unsigned)ceil_div(n_tiles, tpb));
+    EMSIM_LAUNCH(N, (k_col_fft_mul_ifft<N, Mul>), grid, dim3(Cfg::BLOCK), 0, s, data, ld, ncols,
+                 tiles_per_img, img_stride, n_tiles, (int)tpb, mul, tw);
     return EMSIM_HIP_OK;
 }
 
@@ -288,19 +339,30 @@ template <int N>
 int launch_r2c_pairs(const float* in, float2* out, int64_t n_pairs, int rows_per_plane,
                      int planes_per_group, int64_t in_group_stride, int64_t in_plane0, int ldh,
                      const float2* tw, hipStream_t s) {
-    using Cfg = RowCfg<N>;
-    EMSIM_LAUNCH(N, (k_row_r2c_pairs<N>), dim3((unsigned)ceil_div(n_pairs, Cfg::ROWS)),
-                       dim3(Cfg::BLOCK), 0, s, in, out, n_pairs, rows_per_plane, planes_per_group,
-                       in_group_stride, in_plane0, ldh, tw);
+    int rpw;
+    dim3 grid;
+    row_grid<N>(n_pairs, rpw, grid);
+    OpR2CPairs<N> op;
+    op.in = in;
+    op.out = out;
+    op.rows_per_plane = rows_per_plane;
+    op.planes_per_group = planes_per_group;
+    op.ldh = ldh;
+    op.in_group_stride = in_group_stride;
+    op.in_plane0 = in_plane0;
+    EMSIM_LAUNCH(N, (k_row_r2c_pairs<N>), grid, dim3(RowCfg<N>::BLOCK), 0, s, op, n_pairs, rpw,
+                 tw);
     return EMSIM_HIP_OK;
 }
 
 template <int N>
 int launch_c2r_pairs(const float2* in, float* out, int64_t n_pairs, float clip_lo, int ldh,
                      const float2* tw, hipStream_t s) {
-    using Cfg = RowCfg<N>;
-    EMSIM_LAUNCH(N, (k_row_c2r_pairs<N>), dim3((unsigned)ceil_div(n_pairs, Cfg::ROWS)),
-                       dim3(Cfg::BLOCK), 0, s, in, out, n_pairs, clip_lo, ldh, tw);
+    int rpw;
+    dim3 grid;
+    row_grid<N>(n_pairs, rpw, grid);
+    EMSIM_LAUNCH(N, (k_row_c2r_pairs<N>), grid, dim3(RowCfg<N>::BLOCK), 0, s,
+                 (OpC2RPairs<N>{in, out, clip_lo, ldh}), n_pairs, rpw, tw);
     return EMSIM_HIP_OK;
 }
 
@@ -887,4 +949,230 @@ extern "C" int emsim_hip_timing_collect(char* buf, size_t cap, size_t* needed) {
         buf[m] = '\0';
     }
     return rc;
+}
+
+// ==========================================================================
+// fused batched path (kernels_fused.hpp)
+// ==========================================================================
+namespace {
+
+template <int N>
+int launch_spec_sparse(const uint32_t* row_start, const uint16_t* bucket, const int2* twq,
+                       const float* sf, float2* Y, int slice, int n_slices, int n_elems, int H,
+                       int ldh, int n2, int bandlimit, int band_r2, float inv_npix, int batch,
+                       const float2* tw, hipStream_t s) {
+    using Cfg = SpecCfg<N>;
+    const dim3 grid((unsigned)ceil_div(H, Cfg::C), (unsigned)batch);
+    EMSIM_LAUNCH(N, (k_spec_sparse<N>), grid, dim3(Cfg::BLOCK), 0, s, row_start, bucket, twq, sf,
+                 Y, slice, n_slices, n_elems, H, ldh, n2, bandlimit, band_r2, inv_npix, tw);
+    return EMSIM_HIP_OK;
+}
+
+template <int N>
+int launch_slice_pass_a(bool first, const float2* Y, float2* wave, int64_t n_pairs, int ldh,
+                        float psi0, float factor, const float2* tw, hipStream_t s) {
+    int rpw;
+    dim3 grid;
+    row_grid<N>(n_pairs, rpw, grid);
+    const dim3 block(RowCfg<N>::BLOCK);
+    if (first)
+        EMSIM_LAUNCH(N, (k_slice_pass_a<N, true>), grid, block, 0, s,
+                     (OpSlicePassA<N, true>{Y, wave, ldh, psi0, factor}), n_pairs, rpw, tw);
+    else
+        EMSIM_LAUNCH(N, (k_slice_pass_a<N, false>), grid, block, 0, s,
+                     (OpSlicePassA<N, false>{Y, wave, ldh, psi0, factor}), n_pairs, rpw, tw);
+    return EMSIM_HIP_OK;
+}
+
+template <int N>
+int launch_col_lensprop(float2* data, int ld, int ncols, int64_t img_stride, int n_img,
+                        LensPropMul mul, const float2* tw, hipStream_t s) {
+    return launch_col_mul_impl<N>(data, ld, ncols, img_stride, n_img, mul, tw, s);
+}
+
+int spec_sparse(int n, const uint32_t* row_start, const uint16_t* bucket, const int2* twq,
+                const float* sf, float2* Y, int slice, int n_slices, int n_elems, int H, int ldh,
+                int n2, int bandlimit, int band_r2, float inv_npix, int batch, const float2* tw,
+                hipStream_t s) {
+    EMSIM_DISPATCH(n, launch_spec_sparse, row_start, bucket, twq, sf, Y, slice, n_slices,
+                   n_elems, H, ldh, n2, bandlimit, band_r2, inv_npix, batch, tw, s);
+}
+int slice_pass_a(int n, bool first, const float2* Y, float2* wave, int64_t n_pairs, int ldh,
+                 float psi0, float factor, const float2* tw, hipStream_t s) {
+    EMSIM_DISPATCH(n, launch_slice_pass_a, first, Y, wave, n_pairs, ldh, psi0, factor, tw, s);
+}
+int col_lensprop(int n, float2* data, int ld, int ncols, int64_t img_stride, int n_img,
+                 LensPropMul mul, const float2* tw, hipStream_t s) {
+    EMSIM_DISPATCH(n, launch_col_lensprop, data, ld, ncols, img_stride, n_img, mul, tw, s);
+}
+
+struct FusedWorkspace {
+    int64_t* mol_off;
+    int64_t* elem_off;
+    int32_t* atom_row;
+    uint16_t* atom_y;
+    uint16_t* bucket;
+    uint32_t* row_count;
+    uint32_t* row_start;
+    uint32_t* block_sums;
+    float2* Y;
+    float2* wave;
+    int64_t n_rows;
+    size_t bytes;
+};
+
+FusedWorkspace fused_layout(const emsim_hip_slice_builder* b, int batch, int64_t n_atoms,
+                            bool own_wave, char* base) {
+    FusedWorkspace w;
+    w.n_rows = (int64_t)batch * b->n_slices * b->n_elems * b->n1;
+    const int64_t n_scan = w.n_rows + 1;
+    const int64_t n_blocks = ceil_div(n_scan, kScanTile);
+    const int ldh = half_ld(b->H);
+    size_t off = 0;
+    w.mol_off = (int64_t*)(base + off);    off += align_up(sizeof(int64_t) * (size_t)(batch + 1));
+    w.elem_off = (int64_t*)(base + off);
+    off += align_up(sizeof(int64_t) * (size_t)batch * (b->n_elems + 1));
+    w.atom_row = (int32_t*)(base + off);   off += align_up(sizeof(int32_t) * (size_t)n_atoms);
+    w.atom_y = (uint16_t*)(base + off);    off += align_up(sizeof(uint16_t) * (size_t)n_atoms);
+    w.bucket = (uint16_t*)(base + off);    off += align_up(sizeof(uint16_t) * (size_t)n_atoms);
+    w.row_count = (uint32_t*)(base + off); off += align_up(sizeof(uint32_t) * (size_t)n_scan);
+    w.row_start = (uint32_t*)(base + off); off += align_up(sizeof(uint32_t) * (size_t)n_scan);
+    w.block_sums = (uint32_t*)(base + off); off += align_up(sizeof(uint32_t) * (size_t)n_blocks);
+    w.Y = (float2*)(base + off);
+    off += align_up(sizeof(float2) * (size_t)batch * b->n1 * ldh);
+    w.wave = nullptr;
+    if (own_wave) {
+        w.wave = (float2*)(base + off);
+        off += align_up(sizeof(float2) * (size_t)batch * b->n1 * b->n2);
+    }
+    w.bytes = off;
+    return w;
+}
+
+}  // namespace
+
+extern "C" size_t emsim_hip_simulate_workspace_size(const emsim_hip_slice_builder* b, int batch,
+                                                    int64_t n_atoms, int mode) {
+    if (!b || batch < 1 || n_atoms < 0) return 0;
+    return fused_layout(b, batch, n_atoms, mode == 0, nullptr).bytes;
+}
+
+extern "C" int emsim_hip_simulate(const emsim_hip_slice_builder* b,
+                                  const emsim_hip_wave_propagator* w, const float* coords,
+                                  int64_t n_atoms, const int64_t* mol_off_host,
+                                  const int64_t* elem_counts_host, int batch, float psi0,
+                                  float cs_mm, float defocus, float aperture,
+                                  int symmetric_bandlimit, float* out, int mode, void* workspace,
+                                  size_t workspace_bytes, emsim_stream_t stream_) {
+    hipStream_t s = (hipStream_t)stream_;
+    if (!b || !w || !mol_off_host || !elem_counts_host || !out || batch < 1 || n_atoms < 0 ||
+        (n_atoms > 0 && !coords) || (mode != 0 && mode != 1))
+        return fail(EMSIM_HIP_EINVAL, "simulate: bad arguments");
+    if (b->one_slice)
+        return fail(EMSIM_HIP_EINVAL, "simulate: needs a multi-slice builder");
+    if (w->n1 != b->n1 || w->n2 != b->n2)
+        return fail(EMSIM_HIP_EINVAL, "simulate: builder grid %dx%d != propagator grid %dx%d",
+                    b->n1, b->n2, w->n1, w->n2);
+    if (symmetric_bandlimit && b->n1 != b->n2)
+        return fail(EMSIM_HIP_EINVAL, "simulate: the symmetric band-limit needs a square grid");
+    if (batch > 65535) return fail(EMSIM_HIP_EUNSUPPORTED, "simulate: batch <= 65535");
+    if (n_atoms >= (int64_t)0x7fffffff)
+        return fail(EMSIM_HIP_EUNSUPPORTED, "simulate: at most 2^31-1 atoms per call");
+    const int64_t n_rows_total = (int64_t)batch * b->n_slices * b->n_elems * b->n1;
+    if (n_rows_total >= (int64_t)0x7fffffff)
+        return fail(EMSIM_HIP_EUNSUPPORTED, "simulate: batch * slices * elements * n1 too large");
+    // host-side offsets
+    if (mol_off_host[0] != 0 || mol_off_host[batch] != n_atoms)
+        return fail(EMSIM_HIP_EINVAL, "simulate: molecule offsets must run from 0 to n_atoms");
+    std::vector<int64_t> elem_off((size_t)batch * (b->n_elems + 1));
+    for (int m = 0; m < batch; ++m) {
+        if (mol_off_host[m + 1] < mol_off_host[m])
+            return fail(EMSIM_HIP_EINVAL, "simulate: molecule offsets must not decrease");
+        int64_t o = mol_off_host[m];
+        elem_off[(size_t)m * (b->n_elems + 1)] = o;
+        for (int e = 0; e < b->n_elems; ++e) {
+            const int64_t c = elem_counts_host[(size_t)m * b->n_elems + e];
+            if (c < 0) return fail(EMSIM_HIP_EINVAL, "simulate: negative element count");
+            o += c;
+            elem_off[(size_t)m * (b->n_elems + 1) + e + 1] = o;
+        }
+        if (o > mol_off_host[m + 1])
+            return fail(EMSIM_HIP_EINVAL, "simulate: molecule %d's element counts exceed its atoms",
+                        m);
+    }
+    FusedWorkspace ws = fused_layout(b, batch, n_atoms, mode == 0, (char*)workspace);
+    if (!workspace || workspace_bytes < ws.bytes)
+        return fail(EMSIM_HIP_EWORKSPACE, "simulate: workspace needs %zu bytes", ws.bytes);
+    const int2* twq = nullptr;
+    int rc = get_twiddles_q30(&twq);
+    if (rc) return rc;
+    EMSIM_HIP_CHECK(hipMemcpyAsync(ws.mol_off, mol_off_host, sizeof(int64_t) * (batch + 1),
+                                   hipMemcpyHostToDevice, s));
+    EMSIM_HIP_CHECK(hipMemcpyAsync(ws.elem_off, elem_off.data(), sizeof(int64_t) * elem_off.size(),
+                                   hipMemcpyHostToDevice, s));
+
+    BatchGeom bg;
+    bg.g.e0 = b->edges;
+    bg.g.e1 = b->edges + (b->n_slices + 1);
+    bg.g.e2 = bg.g.e1 + (b->n1 + 1);
+    bg.g.n0 = b->n_slices;
+    bg.g.n1 = b->n1;
+    bg.g.n2 = b->n2;
+    bg.g.last0 = b->last_exact[0];
+    bg.g.last1 = b->last_exact[1];
+    bg.g.last2 = b->last_exact[2];
+    bg.g.inv0 = b->inv_d[0];
+    bg.g.inv1 = b->inv_d[1];
+    bg.g.inv2 = b->inv_d[2];
+    bg.g.one_slice = 0;
+    bg.g.tile_rows = 1;
+    bg.g.tiles_per_plane = b->n1;
+    bg.n_elems = b->n_elems;
+    bg.batch = batch;
+
+    // 1. bucket the atoms by (molecule, slice, element, row)
+    const int64_t n_scan = ws.n_rows + 1;
+    const int64_t n_blocks = ceil_div(n_scan, kScanTile);
+    EMSIM_LAUNCH(0, k_fill_u32, dim3(grid_for(n_scan, 256)), dim3(256), 0, s, ws.row_count,
+                 n_scan, 0u);
+    if (n_atoms > 0)
+        EMSIM_LAUNCH(0, k_csr_classify, dim3(grid_for(n_atoms, 256)), dim3(256), 0, s, coords,
+                     n_atoms, ws.mol_off, ws.elem_off, bg, ws.atom_row, ws.atom_y, ws.row_count);
+    EMSIM_LAUNCH(0, k_scan_tiles, dim3((unsigned)n_blocks), dim3(kScanBlock), 0, s, ws.row_count,
+                 ws.row_start, n_scan, ws.block_sums);
+    EMSIM_LAUNCH(0, k_scan_sums, dim3(1), dim3(kScanBlock), 0, s, ws.block_sums, n_blocks);
+    EMSIM_LAUNCH(0, k_scan_add, dim3((unsigned)n_blocks), dim3(kScanBlock), 0, s, ws.row_start,
+                 n_scan, ws.block_sums);
+    if (n_atoms > 0)
+        EMSIM_LAUNCH(0, k_csr_scatter, dim3(grid_for(n_atoms, 256)), dim3(256), 0, s, n_atoms,
+                     ws.atom_row, ws.atom_y, ws.row_start, ws.row_count, ws.bucket);
+
+    // 2. the slices
+    float2* wave = mode == 0 ? ws.wave : (float2*)out;
+    const int ldh = half_ld(b->H);
+    const int r = (b->n1 < b->n2 ? b->n1 : b->n2) / 2;
+    const float inv_npix = 1.0f / (float)((int64_t)b->n1 * b->n2);
+    const int64_t npix = (int64_t)b->n1 * b->n2;
+    const int64_t rows = (int64_t)batch * b->n1;
+    const PropagatorMul prop = make_propagator(w, (float)b->dz);
+    for (int sl = 0; sl < b->n_slices; ++sl) {
+        rc = spec_sparse(b->n1, ws.row_start, ws.bucket, twq, b->sf, ws.Y, sl, b->n_slices,
+                         b->n_elems, b->H, ldh, b->n2, symmetric_bandlimit ? 1 : 0, r * r,
+                         inv_npix, batch, b->tw, s);
+        if (rc) return rc;
+        rc = slice_pass_a(b->n2, sl == 0, ws.Y, wave, rows / 2, ldh, psi0, w->factor, b->tw, s);
+        if (rc) return rc;
+        if (mode == 0 && sl == b->n_slices - 1) {
+            LensPropMul lp;
+            lp.p = prop;
+            lp.l = make_lens(w, cs_mm, defocus, aperture);
+            rc = col_lensprop(b->n1, wave, b->n2, b->n2, npix, batch, lp, b->tw, s);
+        } else {
+            rc = col_prop(b->n1, wave, b->n2, b->n2, npix, batch, prop, b->tw, s);
+        }
+        if (rc) return rc;
+    }
+    // 3. back to real space: |psi|^2 of the image wave, or the exit wave
+    if (mode == 0) return row_ifft_intensity(b->n2, wave, out, rows, b->tw, s);
+    return row_fft(b->n2, true, wave, wave, rows, b->tw, s);
 }

@@ -1,0 +1,276 @@
+// Fused batched path: atoms -> images without dense histograms or
+// per-element spectra in HBM.
+//
+// Per batch of B molecules:
+//   1. k_csr_classify / scan / k_csr_scatter: the atoms' exact bins (same
+//      edge tables and rule as the histogram kernels) are sorted into rows
+//      r = ((b nS + s) nE + e) n1 + x; row r's bucket holds the y bins.
+// Per slice s (three launches for the whole batch):
+//   2. k_spec_sparse (column tiles):  R_e(x, j) = sum_{atoms in row x} w^(j y)
+//      is the row DFT of the element's histogram plane, formed directly from
+//      the bucket; then FFT_col, * F_e, sum over e, * D / nPix, IFFT_col
+//      -> Y (the half spectrum whose C2R is the slice's potential).
+//   3. k_slice_pass_a (row pairs): C2R of two Y rows -> V rows (clip), then
+//      for each of the two wave rows IFFT_row -> * exp(i sigma V) -> FFT_row.
+//   4. pass B (k_col_fft_mul_ifft): FFT_col -> * P / nPix -> IFFT_col; on
+//      the last slice * H P / nPix, which is the lens applied to the exit
+//      wave's spectrum (F F^-1 = nPix), so the final row pass writes |psi|^2.
+//
+// R_e sums run in 64-bit fixed point (twiddles in Q30), so the result does
+// not depend on the order atoms landed in their bucket.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fft_lds.hpp"
+#include "kernels_slice.hpp"
+#include "kernels_wave.hpp"
+
+namespace emsim {
+
+constexpr float kQ30Inv = 9.31322574615478515625e-10f;   // 2^-30
+
+struct BatchGeom {
+    BinGeom g;                 // edge tables and grid (g.n0 = nS)
+    int n_elems;
+    int batch;
+};
+
+// Pass 1: per atom its row and y bin; count atoms per row.  mol_off (B+1)
+// and elem_off (B (nE+1)) are absolute atom offsets of molecules and of each
+// molecule's element groups.
+__global__ void k_csr_classify(const float* __restrict__ coords, int64_t n_atoms,
+                               const int64_t* __restrict__ mol_off,
+                               const int64_t* __restrict__ elem_off, BatchGeom bg,
+                               int32_t* __restrict__ atom_row, uint16_t* __restrict__ atom_y,
+                               uint32_t* __restrict__ row_count) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const BinGeom& g = bg.g;
+    for (int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; a < n_atoms; a += stride) {
+        // molecule: last b with mol_off[b] <= a
+        int lo = 0, hi = bg.batch;   // mol_off[lo] <= a < mol_off[hi]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (mol_off[mid] <= a) lo = mid; else hi = mid;
+        }
+        const int b = lo;
+        const int64_t* eo = elem_off + (int64_t)b * (bg.n_elems + 1);
+        int32_t row = -1;
+        uint16_t yb = 0;
+        if (a >= eo[0] && a < eo[bg.n_elems]) {
+            int e = 0;
+            while (e + 1 < bg.n_elems && a >= eo[e + 1]) ++e;
+            const float z = coords[3 * a + 0], x = coords[3 * a + 1], y = coords[3 * a + 2];
+            const int s = bin_of(z, g.e0, g.n0, g.last0, g.inv0);
+            const int i = bin_of(x, g.e1, g.n1, g.last1, g.inv1);
+            const int j = bin_of(y, g.e2, g.n2, g.last2, g.inv2);
+            if (s >= 0 && i >= 0 && j >= 0) {
+                row = (int32_t)((((int64_t)b * g.n0 + s) * bg.n_elems + e) * g.n1 + i);
+                yb = (uint16_t)j;
+                atomicAdd(&row_count[row], 1u);
+            }
+        }
+        atom_row[a] = row;
+        atom_y[a] = yb;
+    }
+}
+
+__global__ void k_csr_scatter(int64_t n_atoms, const int32_t* __restrict__ atom_row,
+                              const uint16_t* __restrict__ atom_y,
+                              const uint32_t* __restrict__ row_start, uint32_t* row_count,
+                              uint16_t* __restrict__ bucket) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; a < n_atoms; a += stride) {
+        const int32_t row = atom_row[a];
+        if (row < 0) continue;
+        const uint32_t slot = atomicSub(&row_count[row], 1u) - 1u;
+        bucket[row_start[row] + slot] = atom_y[a];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Spectrum of one slice of every molecule, from the atom buckets.
+// grid.x = column tiles of the half spectrum, grid.y = molecules.
+// Y rows are ldh complex apart, images n1 * ldh apart.
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(SpecCfg<N>::BLOCK) void k_spec_sparse(
+    const uint32_t* __restrict__ row_start, const uint16_t* __restrict__ bucket,
+    const int2* __restrict__ twq, const float* __restrict__ sf, float2* __restrict__ Y,
+    int slice, int n_slices, int n_elems, int H, int ldh, int n2, int bandlimit, int band_r2,
+    float inv_npix, const float2* __restrict__ tw) {
+    using Cfg = SpecCfg<N>;
+    constexpr int EPT = Cfg::EPT;
+    __shared__ float lds_re[Cfg::C * Cfg::STRIDE];
+    __shared__ float lds_im[Cfg::C * Cfg::STRIDE];
+    const int c = threadIdx.x % Cfg::C;
+    const int t = threadIdx.x / Cfg::C;
+    const int col = blockIdx.x * Cfg::C + c;
+    const bool valid = col < H;
+    const int b = blockIdx.y;
+    float* re = lds_re + c * Cfg::STRIDE;
+    float* im = lds_im + c * Cfg::STRIDE;
+    const int tw_step = kTwN / n2;
+    const int64_t sf_plane = (int64_t)N * H;
+
+    float2 acc[EPT];
+#pragma unroll
+    for (int m = 0; m < EPT; ++m) acc[m] = make_float2(0.f, 0.f);
+
+    for (int e = 0; e < n_elems; ++e) {
+        const int64_t r0 = (((int64_t)b * n_slices + slice) * n_elems + e) * N;
+        float2 v[EPT];
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) {
+            int64_t sr = 0, si = 0;
+            if (valid) {
+                const int64_t r = r0 + t + m * Cfg::T;
+                const uint32_t beg = row_start[r], end = row_start[r + 1];
+                for (uint32_t k = beg; k < end; ++k) {
+                    const int idx = ((col * (int)bucket[k]) & (n2 - 1)) * tw_step;
+                    const int2 w = twq[idx];
+                    sr += w.x;
+                    si += w.y;
+                }
+            }
+            v[m] = make_float2((float)sr * kQ30Inv, (float)si * kQ30Inv);
+        }
+        fft_n<N, EPT, false>(v, re, im, t, tw);
+        const float* f = sf + (int64_t)e * sf_plane + col;
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) {
+            const float fe = valid ? f[(int64_t)(t + m * Cfg::T) * H] : 0.f;
+            acc[m].x += v[m].x * fe;
+            acc[m].y += v[m].y * fe;
+        }
+    }
+    const int kj = freq_int(col, n2);
+#pragma unroll
+    for (int m = 0; m < EPT; ++m) {
+        const int k1 = t + m * Cfg::T;
+        float s = inv_npix;
+        if (bandlimit) {
+            const int ki = freq_int(k1, N);
+            if (ki * ki + kj * kj > band_r2) s = 0.f;
+        }
+        acc[m] = make_float2(acc[m].x * s, acc[m].y * s);
+    }
+    fft_n<N, EPT, true>(acc, re, im, t, tw);
+    if (valid) {
+        float2* dst = Y + (int64_t)b * N * ldh + col;
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) dst[(int64_t)(t + m * Cfg::T) * ldh] = acc[m];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pass A of one slice with the potential's C2R fused in: item = row pair
+// (image b, rows 2q, 2q+1).  FIRST: the incoming wave is the constant psi0.
+// ---------------------------------------------------------------------------
+template <int N, bool FIRST>
+struct OpSlicePassA {
+    static constexpr int T = RowCfg<N>::T, EPT = RowCfg<N>::EPT;
+    const float2* Y;       // (B, n1, ldh) half spectra of this slice
+    float2* wave;          // (B, n1, N), read (unless FIRST) and written in place
+    int ldh;
+    float psi0;
+    float factor;          // lambda * gamma
+    struct Regs {
+        float2 z[EPT];
+        float2 w0[EPT];
+        float2 w1[EPT];
+    };
+    __device__ __forceinline__ void load(int64_t pair, int t, bool ok, Regs& r) const {
+        const float2* ia = Y + pair * 2 * (int64_t)ldh;
+        const float2* ib = ia + ldh;
+        const float2* wa = wave + pair * 2 * (int64_t)N;
+        const float2* wb = wa + N;
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) {
+            const int k = t + m * T;
+            float2 z = make_float2(0.f, 0.f);
+            if (ok) {
+                const int kk = k <= N / 2 ? k : N - k;
+                float2 a = ia[kk], b = ib[kk];
+                if (kk == 0 || kk == N / 2) { a.y = 0.f; b.y = 0.f; }
+                if (k > N / 2) { a.y = -a.y; b.y = -b.y; }
+                z = make_float2(a.x - b.y, a.y + b.x);   // a + i b
+            }
+            r.z[m] = z;
+            if constexpr (!FIRST) {
+                r.w0[m] = ok ? wa[k] : make_float2(0.f, 0.f);
+                r.w1[m] = ok ? wb[k] : make_float2(0.f, 0.f);
+            }
+        }
+    }
+    __device__ __forceinline__ void transmit_row(float2 (&w)[EPT], const float (&V)[EPT], int t,
+                                                 float* re, float* im,
+                                                 const float2* __restrict__ tw) const {
+        if constexpr (FIRST) {
+#pragma unroll
+            for (int m = 0; m < EPT; ++m) w[m] = make_float2(psi0, 0.f);
+        } else {
+            fft_n<N, EPT, true>(w, re, im, t, tw);
+        }
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) {
+            float s, c;
+            fast_sincos(__fmul_rn(V[m], factor), &s, &c);
+            const float2 x = w[m];
+            w[m] = make_float2(x.x * c - x.y * s, x.x * s + x.y * c);
+        }
+        fft_n<N, EPT, false>(w, re, im, t, tw);
+    }
+    __device__ __forceinline__ void run(int64_t pair, int t, bool ok, Regs& r, float* re,
+                                        float* im, const float2* __restrict__ tw) const {
+        fft_n<N, EPT, true>(r.z, re, im, t, tw);
+        float va[EPT], vb[EPT];
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) {
+            va[m] = fmaxf(r.z[m].x, 1e-13f);
+            vb[m] = fmaxf(r.z[m].y, 1e-13f);
+        }
+        transmit_row(r.w0, va, t, re, im, tw);
+        transmit_row(r.w1, vb, t, re, im, tw);
+        if (ok) {
+            float2* wa = wave + pair * 2 * (int64_t)N;
+            float2* wb = wa + N;
+#pragma unroll
+            for (int m = 0; m < EPT; ++m) {
+                wa[t + m * T] = r.w0[m];
+                wb[t + m * T] = r.w1[m];
+            }
+        }
+    }
+};
+
+template <int N, bool FIRST>
+__global__ __launch_bounds__(RowCfg<N>::BLOCK) void k_slice_pass_a(OpSlicePassA<N, FIRST> op,
+                                                                    int64_t n, int rpw,
+                                                                    const float2* tw) {
+    row_worker<N>(op, n, rpw, tw);
+}
+
+// Last slice of the fused path: propagator and lens in one multiplier.
+// H P / nPix with both masks (2/3 propagator mask and aperture).
+struct LensPropMul {
+    PropagatorMul p;
+    LensMul l;
+    static constexpr bool kTable = true;
+    __device__ __forceinline__ void prepare(float2* px, int tid, int nthreads) const {
+        p.prepare(px, tid, nthreads);
+    }
+    __device__ __forceinline__ float2 column(int j) const { return p.column(j); }
+    __device__ __forceinline__ float2 apply(float2 w, int i, int j, const float2* px,
+                                            float2 py) const {
+        // l applies exp(-i chi) / nPix with the aperture; p.apply already
+        // carries 1/nPix, so undo one of them
+        const float2 wp = p.apply(w, i, j, px, py);
+        const float2 wl = l(wp, i, j);
+        const float n = (float)(p.n1 * p.n2);
+        return make_float2(wl.x * n, wl.y * n);
+    }
+};
+
+}  // namespace emsim

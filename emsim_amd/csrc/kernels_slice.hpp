@@ -255,56 +255,63 @@ __global__ void k_add_water(float* hist, int n_elems, int64_t vox, int idx_h, in
 // + in_plane0 so that a chunk of slices of every element can be addressed.
 // ---------------------------------------------------------------------------
 template <int N>
-__global__ __launch_bounds__(RowCfg<N>::BLOCK) void k_row_r2c_pairs(
-    const float* __restrict__ in, float2* __restrict__ out, int64_t n_pairs, int rows_per_plane,
-    int planes_per_group, int64_t in_group_stride, int64_t in_plane0, int ldh,
-    const float2* __restrict__ tw) {
-    using Cfg = RowCfg<N>;
-    constexpr int H = N / 2 + 1;
-    __shared__ float lds_re[Cfg::ROWS * Cfg::WORDS];
-    __shared__ float lds_im[Cfg::ROWS * Cfg::WORDS];
-    const int rib = threadIdx.x / Cfg::T;
-    const int t = threadIdx.x % Cfg::T;
-    const int64_t pair = (int64_t)blockIdx.x * Cfg::ROWS + rib;
-    const bool valid = pair < n_pairs;
-    float* re = lds_re + rib * Cfg::WORDS;
-    float* im = lds_im + rib * Cfg::WORDS;
-    using L = LdsLayout<N>;
+struct OpR2CPairs {
+    static constexpr int T = RowCfg<N>::T, EPT = RowCfg<N>::EPT;
+    static constexpr int H = N / 2 + 1;
+    const float* in;
+    float2* out;
+    int rows_per_plane, planes_per_group, ldh;
+    int64_t in_group_stride, in_plane0;
+    struct Regs { float2 v[EPT]; };
 
-    const int pairs_per_plane = rows_per_plane / 2;
-    const int64_t p = valid ? pair / pairs_per_plane : 0;
-    const int q = valid ? (int)(pair - p * pairs_per_plane) : 0;
-    const int64_t in_plane = (p / planes_per_group) * in_group_stride + (p % planes_per_group) + in_plane0;
-    const float* ra = in + (in_plane * rows_per_plane + 2 * q) * (int64_t)N;
-    const float* rb = ra + N;
-
-    float2 v[Cfg::EPT];
-#pragma unroll
-    for (int m = 0; m < Cfg::EPT; ++m)
-        v[m] = valid ? make_float2(ra[t + m * Cfg::T], rb[t + m * Cfg::T]) : make_float2(0.f, 0.f);
-    fft_n<N, Cfg::EPT, false>(v, re, im, t, tw);
-    // full spectrum Z to LDS, then split
-#pragma unroll
-    for (int m = 0; m < Cfg::EPT; ++m) {
-        const int idx = L::at(t + m * Cfg::T);
-        re[idx] = v[m].x;
-        im[idx] = v[m].y;
+    __device__ __forceinline__ void locate(int64_t pair, int64_t& p, int& q) const {
+        const int pairs_per_plane = rows_per_plane / 2;
+        p = pair / pairs_per_plane;
+        q = (int)(pair - p * pairs_per_plane);
     }
-    __syncthreads();
-    if (valid) {
-        float2* oa = out + (p * rows_per_plane + 2 * q) * (int64_t)ldh;
-        float2* ob = oa + ldh;
-        for (int k = t; k < H; k += Cfg::T) {
-            const int ik = L::at(k & (N - 1));
-            const int inr = L::at((N - k) & (N - 1));
-            const float zr = re[ik], zi = im[ik];
-            const float wr = re[inr], wi = -im[inr];   // conj Z[N-k]
-            oa[k] = make_float2(0.5f * (zr + wr), 0.5f * (zi + wi));
-            // (Z - conj Z[N-k]) / (2i) = -i/2 (Z - W)
-            ob[k] = make_float2(0.5f * (zi - wi), -0.5f * (zr - wr));
+    __device__ __forceinline__ void load(int64_t pair, int t, bool ok, Regs& r) const {
+        int64_t p;
+        int q;
+        locate(ok ? pair : 0, p, q);
+        const int64_t in_plane =
+            (p / planes_per_group) * in_group_stride + (p % planes_per_group) + in_plane0;
+        const float* ra = in + (in_plane * rows_per_plane + 2 * q) * (int64_t)N;
+        const float* rb = ra + N;
+#pragma unroll
+        for (int m = 0; m < EPT; ++m)
+            r.v[m] = ok ? make_float2(ra[t + m * T], rb[t + m * T]) : make_float2(0.f, 0.f);
+    }
+    __device__ __forceinline__ void run(int64_t pair, int t, bool ok, Regs& r, float* re,
+                                        float* im, const float2* __restrict__ tw) const {
+        using L = LdsLayout<N>;
+        fft_n<N, EPT, false>(r.v, re, im, t, tw);
+        // full spectrum Z to LDS, then split
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) {
+            const int idx = L::at(t + m * T);
+            re[idx] = r.v[m].x;
+            im[idx] = r.v[m].y;
         }
+        __syncthreads();
+        if (ok) {
+            int64_t p;
+            int q;
+            locate(pair, p, q);
+            float2* oa = out + (p * rows_per_plane + 2 * q) * (int64_t)ldh;
+            float2* ob = oa + ldh;
+            for (int k = t; k < H; k += T) {
+                const int ik = L::at(k & (N - 1));
+                const int inr = L::at((N - k) & (N - 1));
+                const float zr = re[ik], zi = im[ik];
+                const float wr = re[inr], wi = -im[inr];   // conj Z[N-k]
+                oa[k] = make_float2(0.5f * (zr + wr), 0.5f * (zi + wi));
+                // (Z - conj Z[N-k]) / (2i) = -i/2 (Z - W)
+                ob[k] = make_float2(0.5f * (zi - wi), -0.5f * (zr - wr));
+            }
+        }
+        __syncthreads();   // the LDS is rewritten by the next row's transform
     }
-}
+};
 
 // ---------------------------------------------------------------------------
 // C2R of row pairs with the clip: half rows A, B (H = N/2+1) -> real rows.
@@ -313,45 +320,54 @@ __global__ __launch_bounds__(RowCfg<N>::BLOCK) void k_row_r2c_pairs(
 // IFFT[A + iB] = a + i b.
 // ---------------------------------------------------------------------------
 template <int N>
-__global__ __launch_bounds__(RowCfg<N>::BLOCK) void k_row_c2r_pairs(
-    const float2* __restrict__ in, float* __restrict__ out, int64_t n_pairs,
-    float clip_lo, int ldh, const float2* __restrict__ tw) {
-    using Cfg = RowCfg<N>;
-    __shared__ float lds_re[Cfg::ROWS * Cfg::WORDS];
-    __shared__ float lds_im[Cfg::ROWS * Cfg::WORDS];
-    const int rib = threadIdx.x / Cfg::T;
-    const int t = threadIdx.x % Cfg::T;
-    const int64_t pair = (int64_t)blockIdx.x * Cfg::ROWS + rib;
-    const bool valid = pair < n_pairs;
-    float* re = lds_re + rib * Cfg::WORDS;
-    float* im = lds_im + rib * Cfg::WORDS;
+struct OpC2RPairs {
+    static constexpr int T = RowCfg<N>::T, EPT = RowCfg<N>::EPT;
+    const float2* in;
+    float* out;
+    float clip_lo;
+    int ldh;
+    struct Regs { float2 v[EPT]; };
+    __device__ __forceinline__ void load(int64_t pair, int t, bool ok, Regs& r) const {
+        const float2* ia = in + pair * 2 * (int64_t)ldh;
+        const float2* ib = ia + ldh;
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) {
+            const int k = t + m * T;
+            float2 z = make_float2(0.f, 0.f);
+            if (ok) {
+                const int kk = k <= N / 2 ? k : N - k;
+                float2 a = ia[kk], b = ib[kk];
+                if (kk == 0 || kk == N / 2) { a.y = 0.f; b.y = 0.f; }
+                if (k > N / 2) { a.y = -a.y; b.y = -b.y; }
+                z = make_float2(a.x - b.y, a.y + b.x);   // a + i b
+            }
+            r.v[m] = z;
+        }
+    }
+    __device__ __forceinline__ void run(int64_t pair, int t, bool ok, Regs& r, float* re,
+                                        float* im, const float2* __restrict__ tw) const {
+        fft_n<N, EPT, true>(r.v, re, im, t, tw);
+        if (ok) {
+            float* oa = out + pair * 2 * (int64_t)N;
+            float* ob = oa + N;
+#pragma unroll
+            for (int m = 0; m < EPT; ++m) {
+                oa[t + m * T] = fmaxf(r.v[m].x, clip_lo);
+                ob[t + m * T] = fmaxf(r.v[m].y, clip_lo);
+            }
+        }
+    }
+};
 
-    const float2* ia = in + pair * 2 * (int64_t)ldh;
-    const float2* ib = ia + ldh;
-    float2 v[Cfg::EPT];
-#pragma unroll
-    for (int m = 0; m < Cfg::EPT; ++m) {
-        const int k = t + m * Cfg::T;
-        float2 z = make_float2(0.f, 0.f);
-        if (valid) {
-            const int kk = k <= N / 2 ? k : N - k;
-            float2 a = ia[kk], b = ib[kk];
-            if (kk == 0 || kk == N / 2) { a.y = 0.f; b.y = 0.f; }
-            if (k > N / 2) { a.y = -a.y; b.y = -b.y; }
-            z = make_float2(a.x - b.y, a.y + b.x);   // a + i b
-        }
-        v[m] = z;
-    }
-    fft_n<N, Cfg::EPT, true>(v, re, im, t, tw);
-    if (valid) {
-        float* oa = out + pair * 2 * (int64_t)N;
-        float* ob = oa + N;
-#pragma unroll
-        for (int m = 0; m < Cfg::EPT; ++m) {
-            oa[t + m * Cfg::T] = fmaxf(v[m].x, clip_lo);
-            ob[t + m * Cfg::T] = fmaxf(v[m].y, clip_lo);
-        }
-    }
+template <int N>
+__global__ __launch_bounds__(RowCfg<N>::BLOCK) void k_row_r2c_pairs(OpR2CPairs<N> op, int64_t n,
+                                                                     int rpw, const float2* tw) {
+    row_worker<N>(op, n, rpw, tw);
+}
+template <int N>
+__global__ __launch_bounds__(RowCfg<N>::BLOCK) void k_row_c2r_pairs(OpC2RPairs<N> op, int64_t n,
+                                                                     int rpw, const float2* tw) {
+    row_worker<N>(op, n, rpw, tw);
 }
 
 // ---------------------------------------------------------------------------

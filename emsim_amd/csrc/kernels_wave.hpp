@@ -107,18 +107,19 @@ template <int N>
 struct RowCfg {
     static constexpr int EPT = 8;
     static constexpr int T = N / EPT;                        // threads per row
-    static constexpr int ROWS = T >= 256 ? 1 : 256 / T;      // rows per block
-    static constexpr int BLOCK = T * ROWS;
+    static constexpr int BLOCK = T >= 64 ? T : 64;           // one wave, or one row's threads
+    static constexpr int G = BLOCK / T;                      // rows handled side by side
     static constexpr int WORDS = LdsLayout<N>::kWords;       // floats per row plane
 };
 
 // Column tiles: C adjacent columns per block (16 -> 128-byte contiguous
-// segments per access), EPT values per thread, at most kColMaxBlock threads.
+// segments per access), EPT values per thread, at most EMSIM_COL_MAX_BLOCK
+// threads.
 #ifndef EMSIM_COL_EPT
 #define EMSIM_COL_EPT 8
 #endif
 #ifndef EMSIM_COL_MAX_BLOCK
-#define EMSIM_COL_MAX_BLOCK 1024
+#define EMSIM_COL_MAX_BLOCK 512
 #endif
 template <int N>
 struct ColCfg {
@@ -135,8 +136,8 @@ struct ColCfg {
     static constexpr int BLOCK = T * C;
 };
 
-// Row index of this thread's transform; wave-uniform when a row spans whole
-// waves, so it lives in a scalar register.
+// Index of this thread's transform within the block; wave-uniform when a
+// transform spans whole waves, so it lives in a scalar register.
 template <int T>
 __device__ __forceinline__ int row_in_block() {
     const int r = threadIdx.x / T;
@@ -145,98 +146,167 @@ __device__ __forceinline__ int row_in_block() {
 }
 
 // ---------------------------------------------------------------------------
-// row kernels
+// Row worker.  A block (one wave for N <= 512) runs an Op over rpw groups of
+// G rows, one after another, and issues the global loads of the next group
+// before transforming the current one, so the memory latency of one row is
+// hidden behind the FFTs of the previous one.  An Op provides
+//   struct Regs;                                      per-thread registers
+//   load(item, t, ok, Regs&)                          global -> registers
+//   run(item, t, ok, Regs&, re, im, tw)               transform + store
+// Every thread calls run (it contains block barriers); ok is false for
+// padding items past n_items, which neither load nor store.
 // ---------------------------------------------------------------------------
-#define EMSIM_ROW_PROLOGUE(N)                                                   \
-    using Cfg = RowCfg<N>;                                                      \
-    constexpr int EPT = Cfg::EPT;                                               \
-    __shared__ float lds_re[Cfg::ROWS * Cfg::WORDS];                            \
-    __shared__ float lds_im[Cfg::ROWS * Cfg::WORDS];                            \
-    const int rib = row_in_block<Cfg::T>();                                     \
-    const int t = threadIdx.x % Cfg::T;                                         \
-    const int64_t row = (int64_t)blockIdx.x * Cfg::ROWS + rib;                  \
-    const bool valid = row < n_rows;                                            \
-    float* re = lds_re + rib * Cfg::WORDS;                                      \
-    float* im = lds_im + rib * Cfg::WORDS;
-
-// Plain row transform of n_rows rows of length N (in place allowed).
-template <int N, bool INV>
-__global__ __launch_bounds__(RowCfg<N>::BLOCK) void k_row_fft(const float2* in, float2* out,
-                                                               int64_t n_rows,
-                                                               const float2* __restrict__ tw) {
-    EMSIM_ROW_PROLOGUE(N)
-    float2 v[EPT];
-    const float2* src = in + row * N;
-#pragma unroll
-    for (int m = 0; m < EPT; ++m) v[m] = valid ? src[t + m * Cfg::T] : make_float2(0.f, 0.f);
-    fft_n<N, EPT, INV>(v, re, im, t, tw);
-    if (valid) {
-        float2* dst = out + row * N;
-#pragma unroll
-        for (int m = 0; m < EPT; ++m) dst[t + m * Cfg::T] = v[m];
+template <int N, class Op>
+__device__ __forceinline__ void row_worker(const Op& op, int64_t n_items, int rpw,
+                                           const float2* __restrict__ tw) {
+    using Cfg = RowCfg<N>;
+    __shared__ float lds_re[Cfg::G * Cfg::WORDS];
+    __shared__ float lds_im[Cfg::G * Cfg::WORDS];
+    const int gi = row_in_block<Cfg::T>();
+    const int t = threadIdx.x % Cfg::T;
+    float* re = lds_re + gi * Cfg::WORDS;
+    float* im = lds_im + gi * Cfg::WORDS;
+    const int64_t first = (int64_t)blockIdx.x * rpw * Cfg::G + gi;
+    typename Op::Regs cur, nxt;
+    op.load(first, t, first < n_items, cur);
+    for (int r = 0; r < rpw; ++r) {
+        const int64_t it = first + (int64_t)r * Cfg::G;
+        if (r + 1 < rpw) op.load(it + Cfg::G, t, it + Cfg::G < n_items, nxt);
+        op.run(it, t, it < n_items, cur, re, im, tw);
+        if (r + 1 < rpw) cur = nxt;
     }
 }
+
+// Plain row transform of rows of length N (in place allowed).
+template <int N, bool INV>
+struct OpFFT {
+    static constexpr int T = RowCfg<N>::T, EPT = RowCfg<N>::EPT;
+    const float2* in;
+    float2* out;
+    struct Regs { float2 v[EPT]; };
+    __device__ __forceinline__ void load(int64_t it, int t, bool ok, Regs& r) const {
+        const float2* src = in + it * N;
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) r.v[m] = ok ? src[t + m * T] : make_float2(0.f, 0.f);
+    }
+    __device__ __forceinline__ void run(int64_t it, int t, bool ok, Regs& r, float* re, float* im,
+                                        const float2* __restrict__ tw) const {
+        fft_n<N, EPT, INV>(r.v, re, im, t, tw);
+        if (ok) {
+            float2* dst = out + it * N;
+#pragma unroll
+            for (int m = 0; m < EPT; ++m) dst[t + m * T] = r.v[m];
+        }
+    }
+};
 
 // Multislice pass A: [IFFT_row] -> * (cos, sin)(factor V) -> FFT_row.
 // FIRST: the input is the real-space wave itself (no leading IFFT).
 // Rows belong to images of rows_per_img rows; the slice row of global row r
 // is V + (r / rows_per_img) * v_img_stride + (r % rows_per_img) * N.
 template <int N, bool FIRST>
-__global__ __launch_bounds__(RowCfg<N>::BLOCK) void k_row_transmit_fft(
-    const float2* in, const float* __restrict__ V, float2* out, int64_t n_rows,
-    int rows_per_img, int64_t v_img_stride, float factor, const float2* __restrict__ tw) {
-    EMSIM_ROW_PROLOGUE(N)
-    float2 v[EPT];
-    const float2* src = in + row * N;
+struct OpTransmit {
+    static constexpr int T = RowCfg<N>::T, EPT = RowCfg<N>::EPT;
+    const float2* in;
+    const float* V;
+    float2* out;
+    int rows_per_img;
+    int64_t v_img_stride;
+    float factor;
+    struct Regs {
+        float2 v[EPT];
+        float p[EPT];
+    };
+    __device__ __forceinline__ void load(int64_t it, int t, bool ok, Regs& r) const {
+        const float2* src = in + it * N;
+        const int64_t img = it / rows_per_img;
+        const float* vrow = V + img * v_img_stride + (it - img * rows_per_img) * N;
 #pragma unroll
-    for (int m = 0; m < EPT; ++m) v[m] = valid ? src[t + m * Cfg::T] : make_float2(0.f, 0.f);
-    if constexpr (!FIRST) fft_n<N, EPT, true>(v, re, im, t, tw);
-    if (valid) {
-        const int64_t img = row / rows_per_img;
-        const float* vrow = V + img * v_img_stride + (row - img * rows_per_img) * N;
+        for (int m = 0; m < EPT; ++m) {
+            r.v[m] = ok ? src[t + m * T] : make_float2(0.f, 0.f);
+            r.p[m] = ok ? vrow[t + m * T] : 0.f;
+        }
+    }
+    __device__ __forceinline__ void run(int64_t it, int t, bool ok, Regs& r, float* re, float* im,
+                                        const float2* __restrict__ tw) const {
+        if constexpr (!FIRST) fft_n<N, EPT, true>(r.v, re, im, t, tw);
 #pragma unroll
         for (int m = 0; m < EPT; ++m) {
             float s, c;
-            fast_sincos(__fmul_rn(vrow[t + m * Cfg::T], factor), &s, &c);
-            const float2 w = v[m];
-            v[m] = make_float2(w.x * c - w.y * s, w.x * s + w.y * c);
+            fast_sincos(__fmul_rn(r.p[m], factor), &s, &c);
+            const float2 w = r.v[m];
+            r.v[m] = make_float2(w.x * c - w.y * s, w.x * s + w.y * c);
+        }
+        fft_n<N, EPT, false>(r.v, re, im, t, tw);
+        if (ok) {
+            float2* dst = out + it * N;
+#pragma unroll
+            for (int m = 0; m < EPT; ++m) dst[t + m * T] = r.v[m];
         }
     }
-    fft_n<N, EPT, false>(v, re, im, t, tw);
-    if (valid) {
-        float2* dst = out + row * N;
-#pragma unroll
-        for (int m = 0; m < EPT; ++m) dst[t + m * Cfg::T] = v[m];
-    }
-}
+};
 
 // IFFT_row followed by |psi|^2 (the lens epilogue; the column pass already
 // applied H / nPix).
 template <int N>
-__global__ __launch_bounds__(RowCfg<N>::BLOCK) void k_row_ifft_intensity(
-    const float2* in, float* __restrict__ img_out, int64_t n_rows, const float2* __restrict__ tw) {
-    EMSIM_ROW_PROLOGUE(N)
-    float2 v[EPT];
-    const float2* src = in + row * N;
+struct OpIfftIntensity {
+    static constexpr int T = RowCfg<N>::T, EPT = RowCfg<N>::EPT;
+    const float2* in;
+    float* img_out;
+    struct Regs { float2 v[EPT]; };
+    __device__ __forceinline__ void load(int64_t it, int t, bool ok, Regs& r) const {
+        const float2* src = in + it * N;
 #pragma unroll
-    for (int m = 0; m < EPT; ++m) v[m] = valid ? src[t + m * Cfg::T] : make_float2(0.f, 0.f);
-    fft_n<N, EPT, true>(v, re, im, t, tw);
-    if (valid) {
-        float* dst = img_out + row * N;
-#pragma unroll
-        for (int m = 0; m < EPT; ++m) dst[t + m * Cfg::T] = v[m].x * v[m].x + v[m].y * v[m].y;
+        for (int m = 0; m < EPT; ++m) r.v[m] = ok ? src[t + m * T] : make_float2(0.f, 0.f);
     }
+    __device__ __forceinline__ void run(int64_t it, int t, bool ok, Regs& r, float* re, float* im,
+                                        const float2* __restrict__ tw) const {
+        fft_n<N, EPT, true>(r.v, re, im, t, tw);
+        if (ok) {
+            float* dst = img_out + it * N;
+#pragma unroll
+            for (int m = 0; m < EPT; ++m) dst[t + m * T] = r.v[m].x * r.v[m].x + r.v[m].y * r.v[m].y;
+        }
+    }
+};
+
+// One named kernel per row op (distinct names in profiles).
+template <int N, bool INV>
+__global__ __launch_bounds__(RowCfg<N>::BLOCK) void k_row_fft(OpFFT<N, INV> op, int64_t n,
+                                                               int rpw, const float2* tw) {
+    row_worker<N>(op, n, rpw, tw);
+}
+template <int N, bool FIRST>
+__global__ __launch_bounds__(RowCfg<N>::BLOCK) void k_row_transmit_fft(OpTransmit<N, FIRST> op,
+                                                                        int64_t n, int rpw,
+                                                                        const float2* tw) {
+    row_worker<N>(op, n, rpw, tw);
+}
+template <int N>
+__global__ __launch_bounds__(RowCfg<N>::BLOCK) void k_row_ifft_intensity(OpIfftIntensity<N> op,
+                                                                          int64_t n, int rpw,
+                                                                          const float2* tw) {
+    row_worker<N>(op, n, rpw, tw);
+}
+
+// Rows (or row pairs) per block for the row worker: enough blocks to keep
+// about 2048 waves in flight, at most 16 rows each.
+inline int rows_per_block(int64_t n_groups) {
+    int64_t r = n_groups / 2048;
+    return (int)(r < 1 ? 1 : (r > 16 ? 16 : r));
 }
 
 // ---------------------------------------------------------------------------
 // column kernel: FFT_col -> pointwise Mul(k1, j) -> IFFT_col, in place.
-// grid.x = tiles of C columns, grid.y = images; data is n_img images of
-// N rows x ld complex, image stride img_stride elements; ncols <= ld columns
-// are transformed.
+// n_img images of N rows x ld complex (image stride img_stride); the first
+// ncols columns of each image are transformed in tiles of C columns.  Block
+// b handles tiles b*tpb .. b*tpb+tpb-1 (tiles numbered image-major) and
+// loads the next tile while transforming the current one.
 // ---------------------------------------------------------------------------
 template <int N, class Mul>
 __global__ __launch_bounds__(ColCfg<N>::BLOCK) void k_col_fft_mul_ifft(
-    float2* data, int ld, int ncols, int64_t img_stride, Mul mul, const float2* __restrict__ tw) {
+    float2* data, int ld, int ncols, int tiles_per_img, int64_t img_stride, int64_t n_tiles,
+    int tpb, Mul mul, const float2* __restrict__ tw) {
     using Cfg = ColCfg<N>;
     constexpr int EPT = Cfg::EPT;
     __shared__ float lds_re[Cfg::C * Cfg::STRIDE];
@@ -244,26 +314,51 @@ __global__ __launch_bounds__(ColCfg<N>::BLOCK) void k_col_fft_mul_ifft(
     __shared__ float2 lds_px[Mul::kTable ? N : 1];
     const int c = threadIdx.x % Cfg::C;
     const int t = threadIdx.x / Cfg::C;
-    const int col = blockIdx.x * Cfg::C + c;
-    const bool valid = col < ncols;
-    float2* base = data + (int64_t)blockIdx.y * img_stride + col;
     float* re = lds_re + c * Cfg::STRIDE;
     float* im = lds_im + c * Cfg::STRIDE;
-
-    float2 v[EPT];
-#pragma unroll
-    for (int m = 0; m < EPT; ++m)
-        v[m] = valid ? base[(int64_t)(t + m * Cfg::T) * ld] : make_float2(0.f, 0.f);
     mul.prepare(lds_px, threadIdx.x, Cfg::BLOCK);
     if constexpr (Mul::kTable) __syncthreads();
-    const float2 py = mul.column(col);
-    fft_n<N, EPT, false>(v, re, im, t, tw);
+
+    auto locate = [&](int64_t g, int& col, bool& ok) -> float2* {
+        const int64_t img = g / tiles_per_img;
+        col = (int)(g - img * tiles_per_img) * Cfg::C + c;
+        ok = g < n_tiles && col < ncols;
+        return data + img * img_stride + col;
+    };
+    const int64_t g0 = (int64_t)blockIdx.x * tpb;
+    float2 cur[EPT], nxt[EPT];
+    int col;
+    bool ok;
+    float2* base = locate(g0, col, ok);
 #pragma unroll
-    for (int m = 0; m < EPT; ++m) v[m] = mul.apply(v[m], t + m * Cfg::T, col, lds_px, py);
-    fft_n<N, EPT, true>(v, re, im, t, tw);
-    if (valid) {
+    for (int m = 0; m < EPT; ++m)
+        cur[m] = ok ? base[(int64_t)(t + m * Cfg::T) * ld] : make_float2(0.f, 0.f);
+    for (int r = 0; r < tpb; ++r) {
+        int col_n = 0;
+        bool ok_n = false;
+        float2* base_n = nullptr;
+        if (r + 1 < tpb) {
+            base_n = locate(g0 + r + 1, col_n, ok_n);
 #pragma unroll
-        for (int m = 0; m < EPT; ++m) base[(int64_t)(t + m * Cfg::T) * ld] = v[m];
+            for (int m = 0; m < EPT; ++m)
+                nxt[m] = ok_n ? base_n[(int64_t)(t + m * Cfg::T) * ld] : make_float2(0.f, 0.f);
+        }
+        const float2 py = mul.column(col);
+        fft_n<N, EPT, false>(cur, re, im, t, tw);
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) cur[m] = mul.apply(cur[m], t + m * Cfg::T, col, lds_px, py);
+        fft_n<N, EPT, true>(cur, re, im, t, tw);
+        if (ok) {
+#pragma unroll
+            for (int m = 0; m < EPT; ++m) base[(int64_t)(t + m * Cfg::T) * ld] = cur[m];
+        }
+        if (r + 1 < tpb) {
+#pragma unroll
+            for (int m = 0; m < EPT; ++m) cur[m] = nxt[m];
+            base = base_n;
+            col = col_n;
+            ok = ok_n;
+        }
     }
 }
 

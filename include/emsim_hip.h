@@ -160,6 +160,30 @@ int emsim_hip_lens_intensity(const emsim_hip_wave_propagator* w, const float* wa
 int emsim_hip_intensity(const float* wave_dev, float* out_dev, int64_t n,
                         emsim_stream_t stream);
 
+/* ---- fused batched pipeline --------------------------------------------- */
+/* Replaces, for a batch of molecules, the per-image sequence of
+ * emsim/pipe.py:46-84 (build_multi_slices -> init_wave ->
+ * multislice_propagate -> lens_propagate -> |psi|^2) as driven per molecule
+ * by emsim/simulator.py:71-91, with the slice count fixed by the builder
+ * (pot.build_multi_slices(..., n_slices=...)).  No dense histogram,
+ * per-element spectrum or potential slice is written to memory.
+ * coords_dev: (n_atoms, 3) float32 (z, x, y) of all molecules, molecule m's
+ * atoms at [mol_off_host[m], mol_off_host[m+1]), sorted by element;
+ * elem_counts_host: (batch, n_elems) atoms per element of each molecule, in
+ * the builder's element order; psi0: the incident plane-wave amplitude
+ * (sqrt(dose d1 d2), wave_propagator_base.py:25-29).
+ * mode 0: out_dev = (batch, n1, n2) float32 images (lens and |psi|^2 with
+ * cs_mm, defocus, aperture); mode 1: out_dev = (batch, n1, n2) complex64 exit
+ * waves (the lens arguments are ignored). */
+size_t emsim_hip_simulate_workspace_size(const emsim_hip_slice_builder* b, int batch,
+                                         int64_t n_atoms, int mode);
+int emsim_hip_simulate(const emsim_hip_slice_builder* b, const emsim_hip_wave_propagator* w,
+                       const float* coords_dev, int64_t n_atoms, const int64_t* mol_off_host,
+                       const int64_t* elem_counts_host, int batch, float psi0, float cs_mm,
+                       float defocus, float aperture, int symmetric_bandlimit, float* out_dev,
+                       int mode, void* workspace_dev, size_t workspace_bytes,
+                       emsim_stream_t stream);
+
 /* ---- instrumentation (no counterpart in the reference, which has no
  * profiler; simulator.py:77,89-91 only logs wall time per image) ---------- */
 /* While enabled, every kernel launch is bracketed by two HIP events on its
