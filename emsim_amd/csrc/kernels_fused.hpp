@@ -94,8 +94,19 @@ __global__ void k_csr_scatter(int64_t n_atoms, const int32_t* __restrict__ atom_
 // grid.x = column tiles of the half spectrum, grid.y = molecules.
 // Y rows are ldh complex apart, images n1 * ldh apart.
 // ---------------------------------------------------------------------------
+// The (molecule, slice)'s row table (n_elems * N + 1 entries, when it fits)
+// and up to kSpecBucketLds atom y-bins are staged in LDS once per block;
+// anything larger is read from global memory.
+constexpr int kSpecRowsLds = 4097;
+constexpr int kSpecBucketLds = 8192;
+// minimum waves per SIMD the compiler must leave room for (4: <= 128 VGPRs,
+// two 512-thread blocks per CU)
+#ifndef EMSIM_SPEC_MIN_WAVES
+#define EMSIM_SPEC_MIN_WAVES 4
+#endif
+
 template <int N>
-__global__ __launch_bounds__(SpecCfg<N>::BLOCK) void k_spec_sparse(
+__global__ __launch_bounds__(SpecCfg<N>::BLOCK, EMSIM_SPEC_MIN_WAVES) void k_spec_sparse(
     const uint32_t* __restrict__ row_start, const uint16_t* __restrict__ bucket,
     const int2* __restrict__ twq, const float* __restrict__ sf, float2* __restrict__ Y,
     int slice, int n_slices, int n_elems, int H, int ldh, int n2, int bandlimit, int band_r2,
@@ -104,6 +115,8 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK) void k_spec_sparse(
     constexpr int EPT = Cfg::EPT;
     __shared__ float lds_re[Cfg::C * Cfg::STRIDE];
     __shared__ float lds_im[Cfg::C * Cfg::STRIDE];
+    __shared__ uint32_t lds_rows[kSpecRowsLds];
+    __shared__ uint16_t lds_bucket[kSpecBucketLds];
     const int c = threadIdx.x % Cfg::C;
     const int t = threadIdx.x / Cfg::C;
     const int col = blockIdx.x * Cfg::C + c;
@@ -114,22 +127,35 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK) void k_spec_sparse(
     const int tw_step = kTwN / n2;
     const int64_t sf_plane = (int64_t)N * H;
 
+    // stage this (molecule, slice)'s rows (rows of all elements are adjacent)
+    const int64_t rbase = (((int64_t)b * n_slices + slice) * n_elems) * N;
+    const int n_rows = n_elems * N;
+    const bool rows_lds = n_rows < kSpecRowsLds;
+    if (rows_lds)
+        for (int i = threadIdx.x; i <= n_rows; i += Cfg::BLOCK) lds_rows[i] = row_start[rbase + i];
+    __syncthreads();
+    auto row_at = [&](int i) -> uint32_t { return rows_lds ? lds_rows[i] : row_start[rbase + i]; };
+    const uint32_t kbase = row_at(0);
+    const uint32_t kcount = row_at(n_rows) - kbase;
+    const uint32_t kstaged = kcount < (uint32_t)kSpecBucketLds ? kcount : (uint32_t)kSpecBucketLds;
+    for (uint32_t i = threadIdx.x; i < kstaged; i += Cfg::BLOCK) lds_bucket[i] = bucket[kbase + i];
+    __syncthreads();
+
     float2 acc[EPT];
 #pragma unroll
     for (int m = 0; m < EPT; ++m) acc[m] = make_float2(0.f, 0.f);
 
     for (int e = 0; e < n_elems; ++e) {
-        const int64_t r0 = (((int64_t)b * n_slices + slice) * n_elems + e) * N;
         float2 v[EPT];
 #pragma unroll
         for (int m = 0; m < EPT; ++m) {
             int64_t sr = 0, si = 0;
             if (valid) {
-                const int64_t r = r0 + t + m * Cfg::T;
-                const uint32_t beg = row_start[r], end = row_start[r + 1];
+                const int r = e * N + t + m * Cfg::T;
+                const uint32_t beg = row_at(r) - kbase, end = row_at(r + 1) - kbase;
                 for (uint32_t k = beg; k < end; ++k) {
-                    const int idx = ((col * (int)bucket[k]) & (n2 - 1)) * tw_step;
-                    const int2 w = twq[idx];
+                    const int y = k < kstaged ? (int)lds_bucket[k] : (int)bucket[kbase + k];
+                    const int2 w = twq[((col * y) & (n2 - 1)) * tw_step];
                     sr += w.x;
                     si += w.y;
                 }
