@@ -40,6 +40,9 @@ def parse():
                     help="time the reference-shaped kernels (histogram, make_slices, multislice)")
     ap.add_argument("--no-timing", action="store_true",
                     help="do not bracket launches with HIP events")
+    ap.add_argument("--traffic", default=os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                      "profiles", "traffic_512x100_b64.json"),
+                    help="rocprofv3 FETCH/WRITE summary for roofline.traffic")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 4 device-to-device copies of MIB MiB (a known-size "
                          "reference for HBM byte counters) and exit")
@@ -111,36 +114,79 @@ def kernel_bytes_per_launch(name: str, n: int, cfg) -> float:
     return float("nan")
 
 
-def cpu_baseline(mols, cfg, tem):
-    """CPU oracle (numpy float32 fftw semantics, pocketfft single thread) on one image."""
+def measured_traffic(kernel: str, path: str):
+    """HBM bytes per launch of `kernel` from a rocprofv3 FETCH/WRITE summary
+    (emsim_amd.profiling), used only if it was measured on this exact library."""
+    try:
+        from emsim_amd.profiling import library_sha256
+        data = json.loads(open(path).read())
+        if data["library_sha256"] != library_sha256():
+            return None
+        base = kernel.split("<")[0]
+        for name, rec in data["kernels"].items():
+            if name.split("<")[0].split("(")[0].strip() == base:
+                return round(rec["bytes"])
+    except Exception:
+        return None
+    return None
+
+
+def cpu_baseline(mols, cfg, tem, budget_s: float = 12.0, max_images: int = 32):
+    """CPU restatement of the reference's fftw backend on the host cores.
+
+    oracle/fftw_port.c (C, OpenMP over all threads) runs whole images of the
+    same workload until budget_s has elapsed; without it (no C compiler),
+    the numpy oracle (single thread) runs one image.
+    """
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from oracle import emsim_oracle as orc
     from emsim_amd import atoms as atm
     n, nS, dz, d = cfg["n"], cfg["slices"], cfg["dz"], cfg["pixel"]
     lam, gamma = orc.wavelength_and_gamma(tem.beam_energy_kev)
-    mol = atm.centralize(mols[0])
-    mol, zs, counts = atm.sort_elements_and_count(mol, cfg["elements"])
-    sf = orc.scattering_factors_half(list(zs), n, n, d, d)
+    psi0 = float(np.float32(np.sqrt(tem.electron_dose * d * d)))
+    prepared = []
+    for m in mols[:max_images]:
+        mol, zs, counts = atm.sort_elements_and_count(atm.centralize(m), cfg["elements"])
+        prepared.append((mol.coordinates, counts))
+    sf = orc.scattering_factors_half(cfg["elements"], n, n, d, d)
+    try:
+        from oracle import port
+        port.load()
+    except Exception:
+        port = None
+    if port is not None:
+        done, t0 = 0, time.perf_counter()
+        while done < len(prepared) and (done == 0 or time.perf_counter() - t0 < budget_s):
+            coords, counts = prepared[done]
+            img = port.simulate(coords, counts, nS, n, dz, d, sf, lam, gamma, tem.cs_mm,
+                                tem.defocus, tem.aperture, psi0)
+            assert np.isfinite(img).all()
+            done += 1
+        dt = time.perf_counter() - t0
+        return {"value": done / dt, "unit": "images/s", "cores": port.threads(), "kind": "port",
+                "sample": f"{done} images of the workload ({n}x{n}, all {nS} slices) through "
+                          f"oracle/fftw_port.c (C/OpenMP restatement of the fftw backend, own "
+                          f"radix-2 FFT: FFTW is not installed), {dt:.1f} s"}
+    coords, counts = prepared[0]
     t0 = time.perf_counter()
-    hist = orc.bin_atoms_multi_slices(mol.coordinates, counts, nS, n, n, dz, d, d)
+    hist = orc.bin_atoms_multi_slices(coords, counts, nS, n, n, dz, d, d)
     v = orc.make_multi_slices_fftw32(hist, sf).astype(np.float32)
-    w = orc.init_wave(n, n, d, d, tem.electron_dose)
-    w = orc.multislice_fftw32(w, v, dz, lam, gamma, d, d)
+    w = orc.multislice_fftw32(orc.init_wave(n, n, d, d, tem.electron_dose), v, dz, lam, gamma,
+                              d, d)
     img = orc.intensity(orc.lens_propagate_fftw32(w, lam, tem.cs_mm, tem.defocus,
                                                   tem.aperture, d, d))
     dt = time.perf_counter() - t0
     assert np.isfinite(img).all()
     return {"value": 1.0 / dt, "unit": "images/s", "cores": 1, "kind": "port",
-            "sample": f"1 image of the workload ({mols[0].label or len(mols[0].elements)} atoms, "
-                      f"{n}x{n}, all {nS} slices), oracle/emsim_oracle.py numpy float32 "
-                      f"restatement of the fftw backend, {dt:.1f} s"}
+            "sample": f"1 image of the workload ({n}x{n}, all {nS} slices) through the numpy "
+                      f"oracle (float32 fftw semantics, single-threaded pocketfft), {dt:.1f} s"}
 
 
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(images shard across GPUs with no collective; see DESIGN.md)", file=sys.stderr)
+              f"(images would shard across GPUs with no collective)", file=sys.stderr)
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
@@ -198,7 +244,8 @@ def main():
         if top is not None:
             roofline = {"bound": "hbm", "achieved": top["alg_GBs"], "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": round(top["alg_GBs"] / HBM_PEAK_GBS, 4),
-                        "traffic": None, "kernel": top["kernel"], "mean_us": top["mean_us"]}
+                        "traffic": measured_traffic(top["kernel"], args.traffic),
+                        "kernel": top["kernel"], "mean_us": top["mean_us"]}
 
     cpu = None
     if not args.no_cpu_baseline:
