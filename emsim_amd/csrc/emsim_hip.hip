@@ -962,9 +962,10 @@ int launch_spec_sparse(const uint32_t* row_start, const uint16_t* bucket, const 
                        int ldh, int n2, int bandlimit, int band_r2, float inv_npix, int batch,
                        const float2* tw, hipStream_t s) {
     using Cfg = SpecCfg<N>;
-    const dim3 grid((unsigned)ceil_div(H, Cfg::C), (unsigned)batch);
+    const dim3 grid((unsigned)(ceil_div(H, Cfg::C) * (int64_t)batch));
     EMSIM_LAUNCH(N, (k_spec_sparse<N>), grid, dim3(Cfg::BLOCK), 0, s, row_start, bucket, twq, sf,
-                 Y, slice, n_slices, n_elems, H, ldh, n2, bandlimit, band_r2, inv_npix, tw);
+                 Y, slice, n_slices, n_elems, H, ldh, n2, batch, bandlimit, band_r2, inv_npix,
+                 tw);
     return EMSIM_HIP_OK;
 }
 

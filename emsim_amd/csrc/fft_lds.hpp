@@ -103,14 +103,66 @@ struct LdsLayout {
     __device__ static __forceinline__ int at(int i) { return i + (i >> 5); }
 };
 
+// Radix of the leading pass (the rest are radix 8).
+constexpr int lead_radix(int n) {
+    return (ilog2(n) % 3 == 1) ? 2 : (ilog2(n) % 3 == 2) ? 4 : 8;
+}
+
+// Per-thread copy of the twiddles of every pass after the leading one (those
+// are all radix 8).  A worker that runs many transforms with the same t (the
+// row and column kernels) loads them once.  fence() makes those loads
+// complete where it is called: inside a loop that prefetches its next tile,
+// the compiler would otherwise count the twiddle loads as possibly pending
+// and, waiting for them, also wait for the prefetch.
+template <int N, int EPT>
+struct TwCache {
+    static constexpr int R0 = lead_radix(N);
+    static constexpr int P = (ilog2(N) - ilog2(R0)) / 3;   // radix-8 passes with twiddles
+    static constexpr int NB = EPT / 8;                    // butterflies per thread and pass
+    float2 w[P > 0 ? P : 1][NB][7];
+    static constexpr int pass_of(int ns) { return (ilog2(ns) - ilog2(R0)) / 3; }
+    __device__ __forceinline__ void load(const float2* __restrict__ tw, int t) {
+        int ns = R0;
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+#pragma unroll
+            for (int q = 0; q < NB; ++q) {
+                const int k = (t + q * (N / EPT)) & (ns - 1);
+#pragma unroll
+                for (int r = 1; r < 8; ++r) w[p][q][r - 1] = tw[k * r * (kTwN / (ns * 8))];
+            }
+            ns *= 8;
+        }
+    }
+    __device__ __forceinline__ void fence() const {
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+#pragma unroll
+            for (int q = 0; q < NB; ++q)
+#pragma unroll
+                for (int r = 0; r < 7; ++r) asm volatile("" : : "v"(w[p][q][r].x), "v"(w[p][q][r].y));
+    }
+};
+
+// Twiddle r of butterfly q (sub-transform index k) of the pass over
+// sub-transforms of length Ns: from the table, or from a thread's cache.
+template <int N, int EPT, int Ns, int R>
+__device__ __forceinline__ float2 tw_get(const float2* __restrict__ tw, int q, int r, int k) {
+    return tw[k * r * (kTwN / (Ns * R))];
+}
+template <int N, int EPT, int Ns, int R>
+__device__ __forceinline__ float2 tw_get(const TwCache<N, EPT>& c, int q, int r, int k) {
+    static_assert(R == 8, "only the radix-8 passes are cached");
+    return c.w[TwCache<N, EPT>::pass_of(Ns)][q][r - 1];
+}
+
 // One Stockham pass: radix R over sub-transforms of length Ns, EPT values
 // per thread.  Before: v[m] = x[t + m N/EPT].  Butterfly j = t + q N/EPT
 // (q < EPT/R) reads x[j + r N/R] (= v[q + r EPT/R]) and writes
 // y[(j/Ns) Ns R + j%Ns + r Ns].
-template <int N, int EPT, int R, int Ns, bool INV>
+template <int N, int EPT, int R, int Ns, bool INV, class TW>
 __device__ __forceinline__ void stockham_pass(float2 (&v)[EPT], float* __restrict__ re,
-                                              float* __restrict__ im, int t,
-                                              const float2* __restrict__ tw) {
+                                              float* __restrict__ im, int t, const TW& tw) {
     constexpr int NB = EPT / R;
     constexpr int S = N / EPT;
     constexpr bool LAST = (Ns * R == N);
@@ -124,36 +176,38 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[EPT], float* __restric
         for (int r = 0; r < R; ++r) a[r] = v[q + r * NB];
         if constexpr (Ns > 1) {
 #pragma unroll
-            for (int r = 1; r < R; ++r) a[r] = mul_tw<INV>(a[r], tw[k * r * (kTwN / (Ns * R))]);
+            for (int r = 1; r < R; ++r) a[r] = mul_tw<INV>(a[r], tw_get<N, EPT, Ns, R>(tw, q, r, k));
         }
         dft<R, INV>(a);
         if constexpr (LAST) {
 #pragma unroll
             for (int r = 0; r < R; ++r) v[q + r * NB] = a[r];
         } else {
-            const int base = (j - k) * R + k;
+            // at(base + r Ns) == at(base) + at(r Ns): base is a multiple of
+            // Ns R plus k < Ns, so adding r Ns never carries across a
+            // 32-word pad boundary; the constant part becomes an immediate
+            // offset of the LDS instruction
+            const int base = L::at((j - k) * R + k);
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const int idx = L::at(base + r * Ns);
-                re[idx] = a[r].x;
-                im[idx] = a[r].y;
+                re[base + L::at(r * Ns)] = a[r].x;
+                im[base + L::at(r * Ns)] = a[r].y;
             }
         }
     }
     if constexpr (!LAST) {
         __syncthreads();
+        // at(t + m S) == at(t) + at(m S) for t < S (S a power of two)
+        const int base = L::at(t);
 #pragma unroll
-        for (int m = 0; m < EPT; ++m) {
-            const int idx = L::at(t + m * S);
-            v[m] = make_float2(re[idx], im[idx]);
-        }
+        for (int m = 0; m < EPT; ++m) v[m] = make_float2(re[base + L::at(m * S)], im[base + L::at(m * S)]);
         __syncthreads();
     }
 }
 
-template <int N, int EPT, int Ns, bool INV>
+template <int N, int EPT, int Ns, bool INV, class TW>
 __device__ __forceinline__ void fft_rest(float2 (&v)[EPT], float* re, float* im, int t,
-                                         const float2* __restrict__ tw) {
+                                         const TW& tw) {
     if constexpr (Ns < N) {
         stockham_pass<N, EPT, 8, Ns, INV>(v, re, im, t, tw);
         fft_rest<N, EPT, Ns * 8, INV>(v, re, im, t, tw);
@@ -162,14 +216,14 @@ __device__ __forceinline__ void fft_rest(float2 (&v)[EPT], float* re, float* im,
 
 // Unnormalised length-N DFT of the values held by the N/EPT threads of one
 // row/column group.  Every thread of the block must call it (it contains
-// block barriers when N > 8).
-template <int N, int EPT, bool INV>
+// block barriers when N > 8).  tw is the twiddle table (const float2*) or
+// this thread's TwCache<N, EPT>.
+template <int N, int EPT, bool INV, class TW>
 __device__ __forceinline__ void fft_n(float2 (&v)[EPT], float* re, float* im, int t,
-                                      const float2* __restrict__ tw) {
+                                      const TW& tw) {
     static_assert(N >= 8 && N <= kTwN && (N & (N - 1)) == 0, "unsupported FFT length");
     static_assert(EPT >= 8 && EPT % 8 == 0 && EPT <= N, "EPT must be a multiple of 8");
-    constexpr int L = ilog2(N);
-    constexpr int R0 = (L % 3 == 1) ? 2 : (L % 3 == 2) ? 4 : 8;
+    constexpr int R0 = lead_radix(N);
     stockham_pass<N, EPT, R0, 1, INV>(v, re, im, t, tw);
     fft_rest<N, EPT, R0, INV>(v, re, im, t, tw);
 }

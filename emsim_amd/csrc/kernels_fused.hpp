@@ -91,14 +91,27 @@ __global__ void k_csr_scatter(int64_t n_atoms, const int32_t* __restrict__ atom_
 
 // ---------------------------------------------------------------------------
 // Spectrum of one slice of every molecule, from the atom buckets.
-// grid.x = column tiles of the half spectrum, grid.y = molecules.
+// One block per (column tile, molecule): a 1-D grid of n_tiles * batch.
 // Y rows are ldh complex apart, images n1 * ldh apart.
 // ---------------------------------------------------------------------------
-// The (molecule, slice)'s row table (n_elems * N + 1 entries, when it fits)
-// and up to kSpecBucketLds atom y-bins are staged in LDS once per block;
-// anything larger is read from global memory.
+// The (molecule, slice)'s row table (n_elems * N + 1 entries, when it fits),
+// up to kSpecBucketLds atom y-bins and, for n2 <= kSpecTwLds, the Q30
+// twiddles are staged in LDS once per block; anything larger is read from
+// global memory.
 constexpr int kSpecRowsLds = 4097;
 constexpr int kSpecBucketLds = 8192;
+constexpr int kSpecTwLds = 1024;
+// The dispatcher hands workgroups to the 8 XCDs round-robin by linear id.
+// Block L therefore runs work item spec_item(L): XCD x gets one contiguous,
+// tile-major range of items, so the blocks of a column tile share that XCD's
+// L2 copy of the tile's form-factor columns instead of each XCD fetching
+// every tile.
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ unsigned spec_item(unsigned L, unsigned total) {
+    const unsigned x = L % kXcds, k = L / kXcds;
+    const unsigned q = total / kXcds, rem = total % kXcds;
+    return x * q + (x < rem ? x : rem) + k;
+}
 // minimum waves per SIMD the compiler must leave room for (4: <= 128 VGPRs,
 // two 512-thread blocks per CU)
 #ifndef EMSIM_SPEC_MIN_WAVES
@@ -109,19 +122,22 @@ template <int N>
 __global__ __launch_bounds__(SpecCfg<N>::BLOCK, EMSIM_SPEC_MIN_WAVES) void k_spec_sparse(
     const uint32_t* __restrict__ row_start, const uint16_t* __restrict__ bucket,
     const int2* __restrict__ twq, const float* __restrict__ sf, float2* __restrict__ Y,
-    int slice, int n_slices, int n_elems, int H, int ldh, int n2, int bandlimit, int band_r2,
-    float inv_npix, const float2* __restrict__ tw) {
+    int slice, int n_slices, int n_elems, int H, int ldh, int n2, int batch, int bandlimit,
+    int band_r2, float inv_npix, const float2* __restrict__ tw) {
     using Cfg = SpecCfg<N>;
     constexpr int EPT = Cfg::EPT;
     __shared__ float lds_re[Cfg::C * Cfg::STRIDE];
     __shared__ float lds_im[Cfg::C * Cfg::STRIDE];
     __shared__ uint32_t lds_rows[kSpecRowsLds];
     __shared__ uint16_t lds_bucket[kSpecBucketLds];
+    __shared__ int2 lds_twq[kSpecTwLds];
+    const unsigned item = spec_item(blockIdx.x, gridDim.x);
+    const int tile = (int)(item / (unsigned)batch);
+    const int b = (int)(item % (unsigned)batch);
     const int c = threadIdx.x % Cfg::C;
     const int t = threadIdx.x / Cfg::C;
-    const int col = blockIdx.x * Cfg::C + c;
+    const int col = tile * Cfg::C + c;
     const bool valid = col < H;
-    const int b = blockIdx.y;
     float* re = lds_re + c * Cfg::STRIDE;
     float* im = lds_im + c * Cfg::STRIDE;
     const int tw_step = kTwN / n2;
@@ -133,6 +149,9 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, EMSIM_SPEC_MIN_WAVES) void k_spe
     const bool rows_lds = n_rows < kSpecRowsLds;
     if (rows_lds)
         for (int i = threadIdx.x; i <= n_rows; i += Cfg::BLOCK) lds_rows[i] = row_start[rbase + i];
+    const bool tw_lds = n2 <= kSpecTwLds;
+    if (tw_lds)
+        for (int i = threadIdx.x; i < n2; i += Cfg::BLOCK) lds_twq[i] = twq[i * tw_step];
     __syncthreads();
     auto row_at = [&](int i) -> uint32_t { return rows_lds ? lds_rows[i] : row_start[rbase + i]; };
     const uint32_t kbase = row_at(0);
@@ -155,7 +174,8 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, EMSIM_SPEC_MIN_WAVES) void k_spe
                 const uint32_t beg = row_at(r) - kbase, end = row_at(r + 1) - kbase;
                 for (uint32_t k = beg; k < end; ++k) {
                     const int y = k < kstaged ? (int)lds_bucket[k] : (int)bucket[kbase + k];
-                    const int2 w = twq[((col * y) & (n2 - 1)) * tw_step];
+                    const int p = (col * y) & (n2 - 1);
+                    const int2 w = tw_lds ? lds_twq[p] : twq[p * tw_step];
                     sr += w.x;
                     si += w.y;
                 }
@@ -202,37 +222,41 @@ struct OpSlicePassA {
     int ldh;
     float psi0;
     float factor;          // lambda * gamma
-    struct Regs {
-        float2 z[EPT];
-        float2 w0[EPT];
-        float2 w1[EPT];
-    };
-    __device__ __forceinline__ void load(int64_t pair, int t, bool ok, Regs& r) const {
+    // raw half-spectrum values of the pair's two rows at this thread's k
+    __device__ __forceinline__ void load_y(int64_t pair, int t, bool ok, float2 (&ya)[EPT],
+                                           float2 (&yb)[EPT]) const {
         const float2* ia = Y + pair * 2 * (int64_t)ldh;
         const float2* ib = ia + ldh;
-        const float2* wa = wave + pair * 2 * (int64_t)N;
-        const float2* wb = wa + N;
 #pragma unroll
         for (int m = 0; m < EPT; ++m) {
             const int k = t + m * T;
-            float2 z = make_float2(0.f, 0.f);
-            if (ok) {
-                const int kk = k <= N / 2 ? k : N - k;
-                float2 a = ia[kk], b = ib[kk];
-                if (kk == 0 || kk == N / 2) { a.y = 0.f; b.y = 0.f; }
-                if (k > N / 2) { a.y = -a.y; b.y = -b.y; }
-                z = make_float2(a.x - b.y, a.y + b.x);   // a + i b
-            }
-            r.z[m] = z;
-            if constexpr (!FIRST) {
-                r.w0[m] = ok ? wa[k] : make_float2(0.f, 0.f);
-                r.w1[m] = ok ? wb[k] : make_float2(0.f, 0.f);
-            }
+            const int kk = k <= N / 2 ? k : N - k;
+            ya[m] = ok ? ia[kk] : make_float2(0.f, 0.f);
+            yb[m] = ok ? ib[kk] : make_float2(0.f, 0.f);
         }
     }
+    __device__ __forceinline__ void load_w(int64_t pair, int row, int t, bool ok,
+                                           float2 (&w)[EPT]) const {
+        const float2* src = wave + (pair * 2 + row) * (int64_t)N;
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) w[m] = ok ? src[t + m * T] : make_float2(0.f, 0.f);
+    }
+    // Hermitian extension of both rows, packed as a + i b for one complex
+    // inverse FFT whose real / imaginary parts are the two rows' C2R.
+    __device__ __forceinline__ void pack(const float2 (&ya)[EPT], const float2 (&yb)[EPT], int t,
+                                         float2 (&z)[EPT]) const {
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) {
+            const int k = t + m * T;
+            float2 a = ya[m], b = yb[m];
+            if (k == 0 || k == N / 2) { a.y = 0.f; b.y = 0.f; }
+            if (k > N / 2) { a.y = -a.y; b.y = -b.y; }
+            z[m] = make_float2(a.x - b.y, a.y + b.x);
+        }
+    }
+    template <class TW>
     __device__ __forceinline__ void transmit_row(float2 (&w)[EPT], const float (&V)[EPT], int t,
-                                                 float* re, float* im,
-                                                 const float2* __restrict__ tw) const {
+                                                 float* re, float* im, const TW& tw) const {
         if constexpr (FIRST) {
 #pragma unroll
             for (int m = 0; m < EPT; ++m) w[m] = make_float2(psi0, 0.f);
@@ -248,34 +272,64 @@ struct OpSlicePassA {
         }
         fft_n<N, EPT, false>(w, re, im, t, tw);
     }
-    __device__ __forceinline__ void run(int64_t pair, int t, bool ok, Regs& r, float* re,
-                                        float* im, const float2* __restrict__ tw) const {
-        fft_n<N, EPT, true>(r.z, re, im, t, tw);
-        float va[EPT], vb[EPT];
+    __device__ __forceinline__ void store_w(int64_t pair, int row, int t, bool ok,
+                                            const float2 (&w)[EPT]) const {
+        if (!ok) return;
+        float2* dst = wave + (pair * 2 + row) * (int64_t)N;
 #pragma unroll
-        for (int m = 0; m < EPT; ++m) {
-            va[m] = fmaxf(r.z[m].x, 1e-13f);
-            vb[m] = fmaxf(r.z[m].y, 1e-13f);
-        }
-        transmit_row(r.w0, va, t, re, im, tw);
-        transmit_row(r.w1, vb, t, re, im, tw);
-        if (ok) {
-            float2* wa = wave + pair * 2 * (int64_t)N;
-            float2* wb = wa + N;
-#pragma unroll
-            for (int m = 0; m < EPT; ++m) {
-                wa[t + m * T] = r.w0[m];
-                wb[t + m * T] = r.w1[m];
-            }
-        }
+        for (int m = 0; m < EPT; ++m) dst[t + m * T] = w[m];
     }
 };
 
+// minimum waves per SIMD pass A must leave room for (caps VGPRs)
+#ifndef EMSIM_PASSA_MIN_WAVES
+#define EMSIM_PASSA_MIN_WAVES 1
+#endif
+// A block (one wave for N <= 512) handles rpw groups of G row pairs.  The
+// loads are staged so that each is in flight behind transforms: row 1's wave
+// during the C2R and row 0's transforms, and the next pair's half spectra
+// and row 0 during row 1's transforms.
 template <int N, bool FIRST>
-__global__ __launch_bounds__(RowCfg<N>::BLOCK) void k_slice_pass_a(OpSlicePassA<N, FIRST> op,
-                                                                    int64_t n, int rpw,
-                                                                    const float2* tw) {
-    row_worker<N>(op, n, rpw, tw);
+__global__ __launch_bounds__(RowCfg<N>::BLOCK, EMSIM_PASSA_MIN_WAVES) void k_slice_pass_a(
+    OpSlicePassA<N, FIRST> op, int64_t n_pairs, int rpw, const float2* tw) {
+    using Cfg = RowCfg<N>;
+    constexpr int EPT = Cfg::EPT;
+    __shared__ float lds_re[Cfg::G * Cfg::WORDS];
+    __shared__ float lds_im[Cfg::G * Cfg::WORDS];
+    const int gi = row_in_block<Cfg::T>();
+    const int t = threadIdx.x % Cfg::T;
+    float* re = lds_re + gi * Cfg::WORDS;
+    float* im = lds_im + gi * Cfg::WORDS;
+    const int64_t first = (int64_t)blockIdx.x * rpw * Cfg::G + gi;
+    TwCache<N, EPT> twc;
+    twc.load(tw, t);
+    float2 ya[EPT], yb[EPT], w0[EPT], w1[EPT];
+    op.load_y(first, t, first < n_pairs, ya, yb);
+    if constexpr (!FIRST) op.load_w(first, 0, t, first < n_pairs, w0);
+    twc.fence();
+    for (int r = 0; r < rpw; ++r) {
+        const int64_t it = first + (int64_t)r * Cfg::G;
+        const bool ok = it < n_pairs;
+        if constexpr (!FIRST) op.load_w(it, 1, t, ok, w1);
+        float2 z[EPT];
+        op.pack(ya, yb, t, z);
+        fft_n<N, EPT, true>(z, re, im, t, twc);
+        float va[EPT], vb[EPT];
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) {
+            va[m] = fmaxf(z[m].x, 1e-13f);
+            vb[m] = fmaxf(z[m].y, 1e-13f);
+        }
+        op.transmit_row(w0, va, t, re, im, twc);
+        op.store_w(it, 0, t, ok, w0);
+        if (r + 1 < rpw) {
+            const int64_t nx = it + Cfg::G;
+            op.load_y(nx, t, nx < n_pairs, ya, yb);
+            if constexpr (!FIRST) op.load_w(nx, 0, t, nx < n_pairs, w0);
+        }
+        op.transmit_row(w1, vb, t, re, im, twc);
+        op.store_w(it, 1, t, ok, w1);
+    }
 }
 
 // Last slice of the fused path: propagator and lens in one multiplier.

@@ -281,8 +281,9 @@ struct OpR2CPairs {
         for (int m = 0; m < EPT; ++m)
             r.v[m] = ok ? make_float2(ra[t + m * T], rb[t + m * T]) : make_float2(0.f, 0.f);
     }
+    template <class TW>
     __device__ __forceinline__ void run(int64_t pair, int t, bool ok, Regs& r, float* re,
-                                        float* im, const float2* __restrict__ tw) const {
+                                        float* im, const TW& tw) const {
         using L = LdsLayout<N>;
         fft_n<N, EPT, false>(r.v, re, im, t, tw);
         // full spectrum Z to LDS, then split
@@ -344,8 +345,9 @@ struct OpC2RPairs {
             r.v[m] = z;
         }
     }
+    template <class TW>
     __device__ __forceinline__ void run(int64_t pair, int t, bool ok, Regs& r, float* re,
-                                        float* im, const float2* __restrict__ tw) const {
+                                        float* im, const TW& tw) const {
         fft_n<N, EPT, true>(r.v, re, im, t, tw);
         if (ok) {
             float* oa = out + pair * 2 * (int64_t)N;

@@ -121,6 +121,10 @@ struct RowCfg {
 #ifndef EMSIM_COL_MAX_BLOCK
 #define EMSIM_COL_MAX_BLOCK 512
 #endif
+// minimum waves per SIMD the column kernels must leave room for (caps VGPRs)
+#ifndef EMSIM_COL_MIN_WAVES
+#define EMSIM_COL_MIN_WAVES 1
+#endif
 template <int N>
 struct ColCfg {
     static constexpr int EPT = (N / EMSIM_COL_EPT) > 256 ? N / 256
@@ -167,12 +171,15 @@ __device__ __forceinline__ void row_worker(const Op& op, int64_t n_items, int rp
     float* re = lds_re + gi * Cfg::WORDS;
     float* im = lds_im + gi * Cfg::WORDS;
     const int64_t first = (int64_t)blockIdx.x * rpw * Cfg::G + gi;
+    TwCache<N, Cfg::EPT> twc;
+    twc.load(tw, t);
     typename Op::Regs cur, nxt;
     op.load(first, t, first < n_items, cur);
+    twc.fence();
     for (int r = 0; r < rpw; ++r) {
         const int64_t it = first + (int64_t)r * Cfg::G;
         if (r + 1 < rpw) op.load(it + Cfg::G, t, it + Cfg::G < n_items, nxt);
-        op.run(it, t, it < n_items, cur, re, im, tw);
+        op.run(it, t, it < n_items, cur, re, im, twc);
         if (r + 1 < rpw) cur = nxt;
     }
 }
@@ -189,8 +196,9 @@ struct OpFFT {
 #pragma unroll
         for (int m = 0; m < EPT; ++m) r.v[m] = ok ? src[t + m * T] : make_float2(0.f, 0.f);
     }
+    template <class TW>
     __device__ __forceinline__ void run(int64_t it, int t, bool ok, Regs& r, float* re, float* im,
-                                        const float2* __restrict__ tw) const {
+                                        const TW& tw) const {
         fft_n<N, EPT, INV>(r.v, re, im, t, tw);
         if (ok) {
             float2* dst = out + it * N;
@@ -227,8 +235,9 @@ struct OpTransmit {
             r.p[m] = ok ? vrow[t + m * T] : 0.f;
         }
     }
+    template <class TW>
     __device__ __forceinline__ void run(int64_t it, int t, bool ok, Regs& r, float* re, float* im,
-                                        const float2* __restrict__ tw) const {
+                                        const TW& tw) const {
         if constexpr (!FIRST) fft_n<N, EPT, true>(r.v, re, im, t, tw);
 #pragma unroll
         for (int m = 0; m < EPT; ++m) {
@@ -259,8 +268,9 @@ struct OpIfftIntensity {
 #pragma unroll
         for (int m = 0; m < EPT; ++m) r.v[m] = ok ? src[t + m * T] : make_float2(0.f, 0.f);
     }
+    template <class TW>
     __device__ __forceinline__ void run(int64_t it, int t, bool ok, Regs& r, float* re, float* im,
-                                        const float2* __restrict__ tw) const {
+                                        const TW& tw) const {
         fft_n<N, EPT, true>(r.v, re, im, t, tw);
         if (ok) {
             float* dst = img_out + it * N;
@@ -304,7 +314,7 @@ inline int rows_per_block(int64_t n_groups) {
 // loads the next tile while transforming the current one.
 // ---------------------------------------------------------------------------
 template <int N, class Mul>
-__global__ __launch_bounds__(ColCfg<N>::BLOCK) void k_col_fft_mul_ifft(
+__global__ __launch_bounds__(ColCfg<N>::BLOCK, EMSIM_COL_MIN_WAVES) void k_col_fft_mul_ifft(
     float2* data, int ld, int ncols, int tiles_per_img, int64_t img_stride, int64_t n_tiles,
     int tpb, Mul mul, const float2* __restrict__ tw) {
     using Cfg = ColCfg<N>;
@@ -326,6 +336,8 @@ __global__ __launch_bounds__(ColCfg<N>::BLOCK) void k_col_fft_mul_ifft(
         return data + img * img_stride + col;
     };
     const int64_t g0 = (int64_t)blockIdx.x * tpb;
+    TwCache<N, EPT> twc;
+    twc.load(tw, t);
     float2 cur[EPT], nxt[EPT];
     int col;
     bool ok;
@@ -333,6 +345,7 @@ __global__ __launch_bounds__(ColCfg<N>::BLOCK) void k_col_fft_mul_ifft(
 #pragma unroll
     for (int m = 0; m < EPT; ++m)
         cur[m] = ok ? base[(int64_t)(t + m * Cfg::T) * ld] : make_float2(0.f, 0.f);
+    twc.fence();
     for (int r = 0; r < tpb; ++r) {
         int col_n = 0;
         bool ok_n = false;
@@ -344,10 +357,10 @@ __global__ __launch_bounds__(ColCfg<N>::BLOCK) void k_col_fft_mul_ifft(
                 nxt[m] = ok_n ? base_n[(int64_t)(t + m * Cfg::T) * ld] : make_float2(0.f, 0.f);
         }
         const float2 py = mul.column(col);
-        fft_n<N, EPT, false>(cur, re, im, t, tw);
+        fft_n<N, EPT, false>(cur, re, im, t, twc);
 #pragma unroll
         for (int m = 0; m < EPT; ++m) cur[m] = mul.apply(cur[m], t + m * Cfg::T, col, lds_px, py);
-        fft_n<N, EPT, true>(cur, re, im, t, tw);
+        fft_n<N, EPT, true>(cur, re, im, t, twc);
         if (ok) {
 #pragma unroll
             for (int m = 0; m < EPT; ++m) base[(int64_t)(t + m * Cfg::T) * ld] = cur[m];
