@@ -65,12 +65,60 @@ _SIGNATURES = {
     "emsim_hip_simulate_workspace_size": (c_size_t, [c_void_p, c_int, c_int64, c_int]),
     "emsim_hip_simulate": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, POINTER(c_int64),
                                    POINTER(c_int64), c_int, c_float, c_float, c_float, c_float,
-                                   c_int, c_void_p, c_int, c_void_p, c_size_t, c_void_p]),
+                                   c_int, c_void_p, c_void_p, c_int, c_void_p, c_size_t,
+                                   c_void_p]),
+    "emsim_hip_gemm": (c_int, [c_void_p, c_void_p]),
+    "emsim_hip_fno_create": (c_int, [POINTER(c_void_p), c_int, c_int, c_int, c_int, c_int]),
+    "emsim_hip_fno_destroy": (None, [c_void_p]),
+    "emsim_hip_fno_param_count": (c_int64, [c_void_p]),
+    "emsim_hip_fno_param_offsets": (c_int, [c_void_p, POINTER(c_int64), c_int]),
+    "emsim_hip_fno_set_params": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "emsim_hip_fno_get_params": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "emsim_hip_fno_get_grads": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "emsim_hip_fno_workspace_size": (c_size_t, [c_void_p, c_int, c_int]),
+    "emsim_hip_fno_pack_input": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_float, c_float,
+                                         c_int, c_void_p, c_void_p]),
+    "emsim_hip_fno_forward": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p,
+                                      c_size_t, c_void_p]),
+    "emsim_hip_fno_backward": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_size_t,
+                                       c_void_p]),
+    "emsim_hip_fno_adam": (c_int, [c_void_p, c_float, c_float, c_float, c_float, c_float, c_int,
+                                   c_void_p]),
+    "emsim_hip_fno_apply": (c_int, [c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_int,
+                                    c_void_p]),
+    "emsim_hip_fno_loss_workspace_size": (c_size_t, [c_int64, c_int]),
+    "emsim_hip_fno_loss": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_float, c_void_p,
+                                   c_void_p, c_void_p, c_size_t, c_void_p]),
     "emsim_hip_timing_enable": (c_int, [c_int]),
     "emsim_hip_timing_collect": (c_int, [c_void_p, c_size_t, POINTER(c_size_t)]),
 }
 
 EXPORTED = tuple(_SIGNATURES)
+
+DT_F32, DT_BF16 = 0, 1
+
+
+class GemmDesc(ctypes.Structure):
+    """``emsim_gemm_desc``: one batched strided GEMM (see include/emsim_hip.h)."""
+    _fields_ = [
+        ("m", c_int64), ("n", c_int64), ("k", c_int64), ("batch", c_int64),
+        ("a", c_void_p), ("a_bs", c_int64), ("a_sm", c_int64), ("a_sk", c_int64),
+        ("a_type", ctypes.c_int32),
+        ("b", c_void_p), ("b_bs", c_int64), ("b_sk", c_int64), ("b_sn", c_int64),
+        ("b_type", ctypes.c_int32),
+        ("k2", c_int64),
+        ("a2", c_void_p), ("a2_bs", c_int64), ("a2_sm", c_int64), ("a2_sk", c_int64),
+        ("a2_type", ctypes.c_int32),
+        ("b2", c_void_p), ("b2_bs", c_int64), ("b2_sk", c_int64), ("b2_sn", c_int64),
+        ("b2_type", ctypes.c_int32),
+        ("c", c_void_p), ("c_bs", c_int64), ("c_sm", c_int64), ("c_sn", c_int64),
+        ("c_type", ctypes.c_int32),
+        ("bias", c_void_p),
+        ("act", ctypes.c_int32),
+        ("z", c_void_p), ("z_type", ctypes.c_int32),
+        ("beta", c_float),
+        ("split_k", c_int64), ("c_split_stride", c_int64),
+    ]
 
 for _name, (_res, _args) in _SIGNATURES.items():
     _fn = getattr(_lib, _name)

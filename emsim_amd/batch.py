@@ -99,7 +99,7 @@ class BatchPipe:
         return t
 
     # -- fused GPU path --------------------------------------------------------
-    def _simulate(self, batch: PreparedBatch, mode: int) -> torch.Tensor:
+    def _simulate(self, batch: PreparedBatch, mode: int, slices_out=None) -> torch.Tensor:
         B = len(batch)
         n_atoms = batch.coords.shape[0]
         if mode == 0:
@@ -115,7 +115,8 @@ class BatchPipe:
                      ptr(batch.coords), n_atoms, batch.mol_off.ctypes.data_as(POINTER(c_int64)),
                      batch.counts.ctypes.data_as(POINTER(c_int64)), B, float(psi0),
                      float(mic.cs_mm), float(mic.defocus), float(mic.aperture),
-                     1 if self.symmetric_bandlimit else 0, ptr(out), mode, ptr(ws), ws.numel(),
+                     1 if self.symmetric_bandlimit else 0, ptr(out),
+                     None if slices_out is None else ptr(slices_out), mode, ptr(ws), ws.numel(),
                      stream_handle())
         return out
 
@@ -126,6 +127,17 @@ class BatchPipe:
     def exit_waves(self, batch: PreparedBatch) -> torch.Tensor:
         """(B, n, n) complex64 exit waves on the GPU."""
         return self._simulate(batch, 1)
+
+    def exit_waves_and_slices(self, batch: PreparedBatch, slices: torch.Tensor = None):
+        """Exit waves plus the projected potential slices the fused path built on the
+        way, (B, n_slices, n, n) float32 (written into ``slices`` if given)."""
+        shape = (len(batch), self.n_slices, self.n, self.n)
+        if slices is None:
+            slices = torch.empty(shape, dtype=torch.float32, device=device())
+        elif tuple(slices.shape) != shape or slices.dtype != torch.float32 or \
+                not slices.is_contiguous():
+            raise ValueError(f"slices must be a contiguous float32 tensor of shape {shape}")
+        return self._simulate(batch, 1, slices), slices
 
     # -- reference-shaped GPU path (cross-check) --------------------------------
     def build_slices(self, batch: PreparedBatch) -> torch.Tensor:

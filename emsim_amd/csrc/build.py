@@ -15,7 +15,7 @@ HERE = Path(__file__).resolve().parent
 PKG = HERE.parent
 OUT_DIR = PKG / "backend" / "hip_ext"
 LIB = OUT_DIR / "libemsim_hip.so"
-SOURCES = [HERE / "emsim_hip.hip"]
+SOURCES = [HERE / "emsim_hip.hip", HERE / "fno.hip"]
 DEPS = SOURCES + sorted(HERE.glob("*.hpp")) + [PKG.parent / "include" / "emsim_hip.h"]
 
 ARCH = os.environ.get("EMSIM_HIP_ARCH", "gfx950")

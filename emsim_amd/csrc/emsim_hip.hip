@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/emsim_hip.h"
+#include "host_common.hpp"
 #include "fft_lds.hpp"
 #include "kernels_slice.hpp"
 #include "kernels_wave.hpp"
@@ -22,7 +23,7 @@
 
 using namespace emsim;
 
-namespace {
+namespace emsim_host {
 
 thread_local std::string g_last_error;
 
@@ -35,13 +36,6 @@ int fail(int code, const char* fmt, ...) {
     g_last_error = buf;
     return code;
 }
-
-#define EMSIM_HIP_CHECK(call)                                                          \
-    do {                                                                                \
-        hipError_t err_ = (call);                                                       \
-        if (err_ != hipSuccess)                                                         \
-            return fail(EMSIM_HIP_EDEVICE, "%s failed: %s", #call, hipGetErrorString(err_)); \
-    } while (0)
 
 // --------------------------------------------------------------------------
 // launches: error check plus optional per-kernel event timing
@@ -92,16 +86,11 @@ void timing_end(hipEvent_t start, const char* name, int n, hipStream_t s) {
     g_timing_records.push_back({name, n, start, e});
 }
 
-#define EMSIM_LAUNCH(tag_n, kernel, grid, block, shmem, stream, ...)                      \
-    do {                                                                                \
-        hipEvent_t tk_ = timing_begin(stream);                                          \
-        hipLaunchKernelGGL(kernel, grid, block, shmem, stream, __VA_ARGS__);            \
-        hipError_t err_ = hipGetLastError();                                            \
-        if (err_ != hipSuccess)                                                         \
-            return fail(EMSIM_HIP_EDEVICE, "launch of %s failed: %s", #kernel,          \
-                        hipGetErrorString(err_));                                       \
-        timing_end(tk_, #kernel, tag_n, stream);                                        \
-    } while (0)
+}  // namespace emsim_host
+
+using namespace emsim_host;
+
+namespace {
 
 // The reference's PI literal (WavePropagator.cpp:10) as float32.
 constexpr float kPi32 = 3.14159265358979324f;
@@ -969,19 +958,26 @@ int launch_spec_sparse(const uint32_t* row_start, const uint16_t* bucket, const 
     return EMSIM_HIP_OK;
 }
 
+// vout (nullable) receives the slice's potential, (B, n_slices, n1, N)
 template <int N>
 int launch_slice_pass_a(bool first, const float2* Y, float2* wave, int64_t n_pairs, int ldh,
-                        float psi0, float factor, const float2* tw, hipStream_t s) {
+                        float psi0, float factor, float* vout, int slice, int n_slices, int n1,
+                        const float2* tw, hipStream_t s) {
     int rpw;
     dim3 grid;
     row_grid<N>(n_pairs, rpw, grid);
     const dim3 block(RowCfg<N>::BLOCK);
+    const int64_t plane = (int64_t)n1 * N;
     if (first)
         EMSIM_LAUNCH(N, (k_slice_pass_a<N, true>), grid, block, 0, s,
-                     (OpSlicePassA<N, true>{Y, wave, ldh, psi0, factor}), n_pairs, rpw, tw);
+                     (OpSlicePassA<N, true>{Y, wave, ldh, psi0, factor, vout, n_slices * plane,
+                                            slice * plane, n1}),
+                     n_pairs, rpw, tw);
     else
         EMSIM_LAUNCH(N, (k_slice_pass_a<N, false>), grid, block, 0, s,
-                     (OpSlicePassA<N, false>{Y, wave, ldh, psi0, factor}), n_pairs, rpw, tw);
+                     (OpSlicePassA<N, false>{Y, wave, ldh, psi0, factor, vout, n_slices * plane,
+                                             slice * plane, n1}),
+                     n_pairs, rpw, tw);
     return EMSIM_HIP_OK;
 }
 
@@ -999,8 +995,10 @@ int spec_sparse(int n, const uint32_t* row_start, const uint16_t* bucket, const 
                    n_elems, H, ldh, n2, bandlimit, band_r2, inv_npix, batch, tw, s);
 }
 int slice_pass_a(int n, bool first, const float2* Y, float2* wave, int64_t n_pairs, int ldh,
-                 float psi0, float factor, const float2* tw, hipStream_t s) {
-    EMSIM_DISPATCH(n, launch_slice_pass_a, first, Y, wave, n_pairs, ldh, psi0, factor, tw, s);
+                 float psi0, float factor, float* vout, int slice, int n_slices, int n1,
+                 const float2* tw, hipStream_t s) {
+    EMSIM_DISPATCH(n, launch_slice_pass_a, first, Y, wave, n_pairs, ldh, psi0, factor, vout,
+                   slice, n_slices, n1, tw, s);
 }
 int col_lensprop(int n, float2* data, int ld, int ncols, int64_t img_stride, int n_img,
                  LensPropMul mul, const float2* tw, hipStream_t s) {
@@ -1063,8 +1061,9 @@ extern "C" int emsim_hip_simulate(const emsim_hip_slice_builder* b,
                                   int64_t n_atoms, const int64_t* mol_off_host,
                                   const int64_t* elem_counts_host, int batch, float psi0,
                                   float cs_mm, float defocus, float aperture,
-                                  int symmetric_bandlimit, float* out, int mode, void* workspace,
-                                  size_t workspace_bytes, emsim_stream_t stream_) {
+                                  int symmetric_bandlimit, float* out, float* slices_out,
+                                  int mode, void* workspace, size_t workspace_bytes,
+                                  emsim_stream_t stream_) {
     hipStream_t s = (hipStream_t)stream_;
     if (!b || !w || !mol_off_host || !elem_counts_host || !out || batch < 1 || n_atoms < 0 ||
         (n_atoms > 0 && !coords) || (mode != 0 && mode != 1))
@@ -1161,7 +1160,8 @@ extern "C" int emsim_hip_simulate(const emsim_hip_slice_builder* b,
                          b->n_elems, b->H, ldh, b->n2, symmetric_bandlimit ? 1 : 0, r * r,
                          inv_npix, batch, b->tw, s);
         if (rc) return rc;
-        rc = slice_pass_a(b->n2, sl == 0, ws.Y, wave, rows / 2, ldh, psi0, w->factor, b->tw, s);
+        rc = slice_pass_a(b->n2, sl == 0, ws.Y, wave, rows / 2, ldh, psi0, w->factor, slices_out,
+                          sl, b->n_slices, b->n1, b->tw, s);
         if (rc) return rc;
         if (mode == 0 && sl == b->n_slices - 1) {
             LensPropMul lp;

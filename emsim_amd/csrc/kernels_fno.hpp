@@ -1,0 +1,456 @@
+// Kernels of the learned coarse-slice correction: a Fourier neural operator
+// (lifting, spectral layers, projection) on channels-last activations, and
+// its backward pass and optimiser.
+//
+// Every dense contraction is one call of k_gemm: a batched, strided GEMM
+// on bf16 MFMA (v_mfma_f32_32x32x16_bf16) with float32 accumulation.
+// Operands are float32 or bf16 in memory and are rounded to bf16 (nearest
+// even) when staged in LDS; an optional second operand pair extends the
+// sum over k (used to fuse the pointwise channel mix with the inverse
+// transform of a spectral layer).  The truncated 2-D DFTs are GEMMs against
+// cos/sin bases, so they run on MFMA too; only the per-mode complex channel
+// mix (k_fno_mix*) and the elementwise kernels are plain VALU code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/emsim_hip.h"
+
+namespace emsim {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+// float32 -> bf16, round to nearest even (NaN stays NaN)
+__device__ __forceinline__ uint16_t f32_to_bf16(float x) {
+    uint32_t u = __float_as_uint(x);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ float ld_elem(const void* p, int64_t off, int type) {
+    return type == EMSIM_DT_BF16 ? bf16_to_f32(((const uint16_t*)p)[off]) : ((const float*)p)[off];
+}
+
+// GELU with the error function, and its derivative
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.f + erff(x * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float x) {
+    return 0.5f * (1.f + erff(x * 0.70710678118654752f)) +
+           x * 0.39894228040143268f * __expf(-0.5f * x * x);
+}
+
+// ---------------------------------------------------------------------------
+// GEMM.  Block tile 64 x 64, k staged 32 at a time; four waves, each owning a
+// 32 x 32 quarter of the tile (one MFMA accumulator of 16 floats per lane).
+// MFMA 32x32x16 operand layout, lane l, element j < 8:
+//   A(row l % 32, k 8 (l / 32) + j),  B(k 8 (l / 32) + j, col l % 32);
+// result element v < 16: D(row 8 (v / 4) + 4 (l / 32) + v % 4, col l % 32).
+// ---------------------------------------------------------------------------
+constexpr int kGemmM = 64, kGemmN = 64, kGemmK = 32, kGemmThreads = 256;
+constexpr int kGemmLdk = kGemmK + 8;   // bf16 per LDS row: 80 bytes, 16-byte aligned rows
+
+// Stage a 64 x 32 tile of op (rows r0.., k k0..) as bf16 rows[r][k] in LDS.
+// (r, k) lives at p + r sr + k sk; outside [0, nr) x [k0, kend) reads 0.
+__device__ __forceinline__ void gemm_stage(uint16_t* __restrict__ lds, const void* p, int type,
+                                           int64_t sr, int64_t sk, int64_t r0, int64_t nr,
+                                           int64_t k0, int64_t kend) {
+    const int t = threadIdx.x;
+    if (sk == 1) {
+        // k contiguous: thread -> (row t / 4, 8 consecutive k)
+        const int r = t >> 2, kc = (t & 3) * 8;
+        const int64_t row = r0 + r;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int64_t kk = k0 + kc + j;
+            const float v = (row < nr && kk < kend) ? ld_elem(p, row * sr + kk, type) : 0.f;
+            lds[r * kGemmLdk + kc + j] = f32_to_bf16(v);
+        }
+    } else {
+        // rows contiguous (or general): thread -> (k t / 8, 8 consecutive rows)
+        const int kq = t >> 3, rc = (t & 7) * 8;
+        const int64_t kk = k0 + kq;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int64_t row = r0 + rc + j;
+            const float v = (row < nr && kk < kend) ? ld_elem(p, row * sr + kk * sk, type) : 0.f;
+            lds[(rc + j) * kGemmLdk + kq] = f32_to_bf16(v);
+        }
+    }
+}
+
+__device__ __forceinline__ void gemm_accumulate(f32x16_t& acc, const emsim_gemm_desc& d,
+                                                const void* a, int at, int64_t a_sm, int64_t a_sk,
+                                                const void* b, int bt, int64_t b_sk, int64_t b_sn,
+                                                int64_t kbeg, int64_t kend, int64_t m0, int64_t n0,
+                                                uint16_t* lds_a, uint16_t* lds_b) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+    for (int64_t k0 = kbeg; k0 < kend; k0 += kGemmK) {
+        __syncthreads();
+        gemm_stage(lds_a, a, at, a_sm, a_sk, m0, d.m, k0, kend);
+        // B tile stored transposed: rows = output columns, k contiguous
+        gemm_stage(lds_b, b, bt, b_sn, b_sk, n0, d.n, k0, kend);
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kGemmK; kk += 16) {
+            const int ko = kk + 8 * (lane >> 5);
+            const bf16x8_t fa = *(const bf16x8_t*)(lds_a + (wm + (lane & 31)) * kGemmLdk + ko);
+            const bf16x8_t fb = *(const bf16x8_t*)(lds_b + (wn + (lane & 31)) * kGemmLdk + ko);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa, fb, acc, 0, 0, 0);
+        }
+    }
+}
+
+// grid: x = m tiles, y = n tiles, z = batch * splits
+__global__ __launch_bounds__(kGemmThreads) void k_gemm(emsim_gemm_desc d) {
+    __shared__ __align__(16) uint16_t lds_a[kGemmM * kGemmLdk];
+    __shared__ __align__(16) uint16_t lds_b[kGemmN * kGemmLdk];
+    const int64_t splits = d.split_k > 0 ? (d.k + d.split_k - 1) / d.split_k : 1;
+    const int64_t z = blockIdx.z;
+    const int64_t bi = z / splits, si = z % splits;
+    const int64_t m0 = (int64_t)blockIdx.x * kGemmM, n0 = (int64_t)blockIdx.y * kGemmN;
+    f32x16_t acc = {};
+    const int64_t kbeg = d.split_k > 0 ? si * d.split_k : 0;
+    const int64_t kend = d.split_k > 0 ? (kbeg + d.split_k < d.k ? kbeg + d.split_k : d.k) : d.k;
+    const char* a = (const char*)d.a + bi * d.a_bs * (d.a_type == EMSIM_DT_BF16 ? 2 : 4);
+    const char* b = (const char*)d.b + bi * d.b_bs * (d.b_type == EMSIM_DT_BF16 ? 2 : 4);
+    gemm_accumulate(acc, d, a, d.a_type, d.a_sm, d.a_sk, b, d.b_type, d.b_sk, d.b_sn, kbeg, kend,
+                    m0, n0, lds_a, lds_b);
+    if (d.k2 > 0 && si == 0) {
+        const char* a2 = (const char*)d.a2 + bi * d.a2_bs * (d.a2_type == EMSIM_DT_BF16 ? 2 : 4);
+        const char* b2 = (const char*)d.b2 + bi * d.b2_bs * (d.b2_type == EMSIM_DT_BF16 ? 2 : 4);
+        gemm_accumulate(acc, d, a2, d.a2_type, d.a2_sm, d.a2_sk, b2, d.b2_type, d.b2_sk, d.b2_sn,
+                        0, d.k2, m0, n0, lds_a, lds_b);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t col = n0 + (w & 1) * 32 + (lane & 31);
+    if (col >= d.n) return;
+    const int64_t cb = d.split_k > 0 ? z * d.c_split_stride : bi * d.c_bs;
+    const float bias = (d.bias && d.split_k == 0) ? d.bias[col] : 0.f;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const int64_t row = m0 + (w >> 1) * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        if (row >= d.m) continue;
+        const int64_t off = cb + row * d.c_sm + col * d.c_sn;
+        float val = acc[v];
+        if (d.split_k == 0) {
+            if (d.beta != 0.f) val += d.beta * ld_elem(d.c, off, d.c_type);
+            val += bias;
+            if (d.z) {
+                if (d.z_type == EMSIM_DT_BF16) ((uint16_t*)d.z)[off] = f32_to_bf16(val);
+                else ((float*)d.z)[off] = val;
+            }
+            if (d.act == 1) val = gelu(val);
+        }
+        if (d.c_type == EMSIM_DT_BF16) ((uint16_t*)d.c)[off] = f32_to_bf16(val);
+        else ((float*)d.c)[off] = val;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Spectral channel mix of one layer, one block per retained mode (k1, k2).
+// PQ (batch, 2 M1, 2 M2 C): rows k1 (cos), M1 + k1 (sin) of the x-transform;
+// columns (ri M2 + k2) C + c, ri = 0 / 1 the real / imaginary part of the
+// y-transform.  S = sum_x exp(-i kx) T:  S_r = P_r + Q_i, S_i = P_i - Q_r.
+// R (M1, M2, C, C) complex.  O = R S per mode, written to
+// Ob (batch, 2 M1, M2 C): rows ri M1 + k1, columns k2 C + co.
+// ---------------------------------------------------------------------------
+template <int C>
+__global__ __launch_bounds__(256) void k_fno_mix(const float* __restrict__ PQ,
+                                                 const float2* __restrict__ R,
+                                                 float* __restrict__ Ob, int batch, int M1,
+                                                 int M2) {
+    __shared__ float2 lds_r[C * C];
+    __shared__ float2 lds_s[256 / C][C];
+    const int k1 = blockIdx.x / M2, k2 = blockIdx.x % M2;
+    const float2* r = R + (int64_t)blockIdx.x * C * C;
+    for (int i = threadIdx.x; i < C * C; i += blockDim.x) lds_r[i] = r[i];
+    const int pq_ld = 2 * M2 * C, ob_ld = M2 * C;
+    const int bl = threadIdx.x / C, co = threadIdx.x % C;
+    constexpr int BPB = 256 / C;   // images per pass
+    for (int b0 = 0; b0 < batch; b0 += BPB) {
+        __syncthreads();
+        {
+            const int b = b0 + bl, ci = co;
+            float2 s = make_float2(0.f, 0.f);
+            if (b < batch) {
+                const float* pq = PQ + (int64_t)b * 2 * M1 * pq_ld;
+                const float* p = pq + (int64_t)k1 * pq_ld;
+                const float* q = pq + (int64_t)(M1 + k1) * pq_ld;
+                const int cr = k2 * C + ci, cim = (M2 + k2) * C + ci;
+                s = make_float2(p[cr] + q[cim], p[cim] - q[cr]);
+            }
+            lds_s[bl][ci] = s;
+        }
+        __syncthreads();
+        const int b = b0 + bl;
+        if (b < batch) {
+            float orr = 0.f, oi = 0.f;
+#pragma unroll 8
+            for (int ci = 0; ci < C; ++ci) {
+                const float2 w = lds_r[ci * C + co], s = lds_s[bl][ci];
+                orr += w.x * s.x - w.y * s.y;
+                oi += w.x * s.y + w.y * s.x;
+            }
+            float* ob = Ob + (int64_t)b * 2 * M1 * ob_ld;
+            ob[(int64_t)k1 * ob_ld + k2 * C + co] = orr;
+            ob[(int64_t)(M1 + k1) * ob_ld + k2 * C + co] = oi;
+        }
+    }
+}
+
+// Backward of k_fno_mix: from dOb (layout of Ob) and PQ, writes dR (M1, M2,
+// C, C complex, overwritten: one block per mode sums over the batch in a
+// fixed order) and dPQ (layout of PQ).
+template <int C>
+__global__ __launch_bounds__(256) void k_fno_mix_bwd(const float* __restrict__ PQ,
+                                                     const float2* __restrict__ R,
+                                                     const float* __restrict__ dOb,
+                                                     float2* __restrict__ dR,
+                                                     float* __restrict__ dPQ, int batch, int M1,
+                                                     int M2) {
+    __shared__ float2 lds_r[C * C];
+    __shared__ float2 lds_s[256 / C][C];
+    __shared__ float2 lds_do[256 / C][C];
+    const int k1 = blockIdx.x / M2, k2 = blockIdx.x % M2;
+    const float2* r = R + (int64_t)blockIdx.x * C * C;
+    for (int i = threadIdx.x; i < C * C; i += blockDim.x) lds_r[i] = r[i];
+    const int pq_ld = 2 * M2 * C, ob_ld = M2 * C;
+    constexpr int BPB = 256 / C;
+    constexpr int PER = (C * C + 255) / 256;   // dR entries per thread
+    float2 gr[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) gr[q] = make_float2(0.f, 0.f);
+    const int bl = threadIdx.x / C, cx = threadIdx.x % C;
+    for (int b0 = 0; b0 < batch; b0 += BPB) {
+        __syncthreads();
+        {
+            const int b = b0 + bl;
+            float2 s = make_float2(0.f, 0.f), g = make_float2(0.f, 0.f);
+            if (b < batch) {
+                const float* pq = PQ + (int64_t)b * 2 * M1 * pq_ld;
+                const float* p = pq + (int64_t)k1 * pq_ld;
+                const float* q = pq + (int64_t)(M1 + k1) * pq_ld;
+                const int cr = k2 * C + cx, cim = (M2 + k2) * C + cx;
+                s = make_float2(p[cr] + q[cim], p[cim] - q[cr]);
+                const float* dob = dOb + (int64_t)b * 2 * M1 * ob_ld;
+                g = make_float2(dob[(int64_t)k1 * ob_ld + k2 * C + cx],
+                                dob[(int64_t)(M1 + k1) * ob_ld + k2 * C + cx]);
+            }
+            lds_s[bl][cx] = s;      // S(b, ci = cx)
+            lds_do[bl][cx] = g;     // dO(b, co = cx)
+        }
+        __syncthreads();
+        // dS(b, ci = cx) = sum_co conj-mix of R and dO
+        const int b = b0 + bl;
+        if (b < batch) {
+            float sr = 0.f, si = 0.f;
+#pragma unroll 8
+            for (int co = 0; co < C; ++co) {
+                const float2 w = lds_r[cx * C + co], g = lds_do[bl][co];
+                sr += w.x * g.x + w.y * g.y;
+                si += -w.y * g.x + w.x * g.y;
+            }
+            float* pq = dPQ + (int64_t)b * 2 * M1 * pq_ld;
+            float* p = pq + (int64_t)k1 * pq_ld;
+            float* q = pq + (int64_t)(M1 + k1) * pq_ld;
+            const int cr = k2 * C + cx, cim = (M2 + k2) * C + cx;
+            p[cr] = sr;
+            q[cim] = sr;
+            p[cim] = si;
+            q[cr] = -si;
+        }
+        // dR(ci, co) += sum over this pass's images, in image order
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = threadIdx.x + q * 256;
+            if (e < C * C) {
+                const int ci = e / C, co = e % C;
+                float2 acc = gr[q];
+                for (int j = 0; j < BPB && b0 + j < batch; ++j) {
+                    const float2 s = lds_s[j][ci], g = lds_do[j][co];
+                    acc.x += s.x * g.x + s.y * g.y;
+                    acc.y += -s.y * g.x + s.x * g.y;
+                }
+                gr[q] = acc;
+            }
+        }
+    }
+    float2* out = dR + (int64_t)blockIdx.x * C * C;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int e = threadIdx.x + q * 256;
+        if (e < C * C) out[e] = gr[q];
+    }
+}
+
+// dz = dv * gelu'(z) (act = 1) or dv (act = 0); z bf16, dv / dz float32
+__global__ void k_fno_act_bwd(const float* __restrict__ dv, const uint16_t* __restrict__ z,
+                              float* __restrict__ dz, int64_t n, int act) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        dz[i] = act ? dv[i] * gelu_grad(bf16_to_f32(z[i])) : dv[i];
+}
+
+// out[j] = sum over i < rows of x[i * cols + j], two stages in a fixed
+// order: k_colsum_partial writes one partial row per block, k_rowsum adds
+// the partials.
+__global__ __launch_bounds__(256) void k_colsum_partial(const float* __restrict__ x, int64_t rows,
+                                                        int cols, float* __restrict__ part) {
+    // block b sums rows [b * chunk, (b + 1) * chunk); thread (r, j) strides rows
+    const int64_t chunk = (rows + gridDim.x - 1) / gridDim.x;
+    const int64_t r0 = (int64_t)blockIdx.x * chunk;
+    const int64_t r1 = r0 + chunk < rows ? r0 + chunk : rows;
+    const int lanes_per_col = 256 / cols;   // cols divides 256
+    const int j = threadIdx.x % cols, rl = threadIdx.x / cols;
+    float s = 0.f;
+    for (int64_t r = r0 + rl; r < r1; r += lanes_per_col) s += x[r * cols + j];
+    __shared__ float red[256];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (rl == 0) {
+        float t = 0.f;
+        for (int q = 0; q < lanes_per_col; ++q) t += red[q * cols + j];
+        part[(int64_t)blockIdx.x * cols + j] = t;
+    }
+}
+
+// out[j] (+)= sum_p part[p * n + j]
+__global__ void k_rowsum(const float* __restrict__ part, int np, int64_t n,
+                         float* __restrict__ out, int accumulate) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        float s = 0.f;
+        for (int p = 0; p < np; ++p) s += part[(int64_t)p * n + j];
+        out[j] = accumulate ? out[j] + s : s;
+    }
+}
+
+// Adam (bias-corrected), then the bf16 mirror of the parameters.
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, uint16_t* __restrict__ p16, int64_t n, float lr,
+                       float b1, float b2, float eps, float c1, float c2, float wd) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const float gi = g[i] + wd * p[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        p[i] = pi;
+        p16[i] = f32_to_bf16(pi);
+    }
+}
+
+__global__ void k_to_bf16(const float* __restrict__ x, uint16_t* __restrict__ y, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        y[i] = f32_to_bf16(x[i]);
+}
+
+// Network input, channels-last bf16 (batch, n, n, cin_pad):
+// [Re psi / psi0, Im psi / psi0, sigma V_1 .. sigma V_ns, 0 ...].
+__global__ void k_fno_pack(const float2* __restrict__ wave, const float* __restrict__ slices,
+                           int ns, int64_t npix, int64_t total, float inv_psi0, float sigma,
+                           uint16_t* __restrict__ x, int cin_pad) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const int64_t b = i / npix, p = i - b * npix;
+        const float2 w = wave[i];
+        uint16_t* xo = x + i * cin_pad;
+        xo[0] = f32_to_bf16(w.x * inv_psi0);
+        xo[1] = f32_to_bf16(w.y * inv_psi0);
+        const float* v = slices + b * ns * npix + p;
+        for (int s = 0; s < ns; ++s) xo[2 + s] = f32_to_bf16(sigma * v[(int64_t)s * npix]);
+        for (int c = 2 + ns; c < cin_pad; ++c) xo[c] = 0;
+    }
+}
+
+// corrected wave = coarse wave + psi0 (out_r + i out_i)
+__global__ void k_fno_unpack(const float2* __restrict__ coarse, const float* __restrict__ out,
+                             float psi0, float2* __restrict__ wave, int64_t total) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const float2 c = coarse[i];
+        wave[i] = make_float2(c.x + psi0 * out[2 * i], c.y + psi0 * out[2 * i + 1]);
+    }
+}
+
+// Per-image partial sums of |pred - target|^2, |target|^2 and |pred|^2
+// (block p of image b covers pixels [p chunk, (p + 1) chunk)); part is
+// (batch, nparts, 3).
+__global__ __launch_bounds__(256) void k_fno_loss_partial(const float2* __restrict__ pred,
+                                                          const float2* __restrict__ target,
+                                                          int64_t npix, int nparts,
+                                                          float* __restrict__ part) {
+    const int b = blockIdx.y, pidx = blockIdx.x;
+    const int64_t chunk = (npix + nparts - 1) / nparts;
+    const int64_t p0 = (int64_t)pidx * chunk;
+    const int64_t p1 = p0 + chunk < npix ? p0 + chunk : npix;
+    float e = 0.f, t2 = 0.f, a2 = 0.f;
+    for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
+        const float2 a = pred[b * npix + p], t = target[b * npix + p];
+        const float dx = a.x - t.x, dy = a.y - t.y;
+        e += dx * dx + dy * dy;
+        t2 += t.x * t.x + t.y * t.y;
+        a2 += a.x * a.x + a.y * a.y;
+    }
+    __shared__ float re[256], rt[256], ra[256];
+    re[threadIdx.x] = e;
+    rt[threadIdx.x] = t2;
+    ra[threadIdx.x] = a2;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            re[threadIdx.x] += re[threadIdx.x + s];
+            rt[threadIdx.x] += rt[threadIdx.x + s];
+            ra[threadIdx.x] += ra[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        float* o = part + ((int64_t)b * nparts + pidx) * 3;
+        o[0] = re[0];
+        o[1] = rt[0];
+        o[2] = ra[0];
+    }
+}
+
+// loss (batch, 3): |pred - target|^2 / |target|^2, |target|^2, |pred|^2 per
+// image; the mean of column 0 over the batch is the training loss.
+__global__ void k_fno_loss_finish(const float* __restrict__ part, int nparts, int batch,
+                                  float* __restrict__ loss) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= batch) return;
+    float e = 0.f, t = 0.f, a = 0.f;
+    for (int p = 0; p < nparts; ++p) {
+        const float* o = part + ((int64_t)b * nparts + p) * 3;
+        e += o[0];
+        t += o[1];
+        a += o[2];
+    }
+    loss[3 * b] = e / t;
+    loss[3 * b + 1] = t;
+    loss[3 * b + 2] = a;
+}
+
+// dout = d(mean loss) / d(network output) for pred = coarse + psi0 out:
+// scale = 2 psi0 / batch.
+__global__ void k_fno_loss_grad(const float2* __restrict__ pred, const float2* __restrict__ target,
+                                const float* __restrict__ loss, int64_t npix, int64_t total,
+                                float scale, float* __restrict__ dout) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const int64_t b = i / npix;
+        const float k = scale / loss[3 * b + 1];
+        const float2 a = pred[i], t = target[i];
+        dout[2 * i] = k * (a.x - t.x);
+        dout[2 * i + 1] = k * (a.y - t.y);
+    }
+}
+
+}  // namespace emsim

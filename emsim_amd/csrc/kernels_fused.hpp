@@ -113,13 +113,13 @@ __device__ __forceinline__ unsigned spec_item(unsigned L, unsigned total) {
     return x * q + (x < rem ? x : rem) + k;
 }
 // minimum waves per SIMD the compiler must leave room for (4: <= 128 VGPRs,
-// two 512-thread blocks per CU)
+// two 512-thread blocks per CU; larger N fit only one block in LDS anyway)
 #ifndef EMSIM_SPEC_MIN_WAVES
 #define EMSIM_SPEC_MIN_WAVES 4
 #endif
 
 template <int N>
-__global__ __launch_bounds__(SpecCfg<N>::BLOCK, EMSIM_SPEC_MIN_WAVES) void k_spec_sparse(
+__global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_SPEC_MIN_WAVES : 1) void k_spec_sparse(
     const uint32_t* __restrict__ row_start, const uint16_t* __restrict__ bucket,
     const int2* __restrict__ twq, const float* __restrict__ sf, float2* __restrict__ Y,
     int slice, int n_slices, int n_elems, int H, int ldh, int n2, int batch, int bandlimit,
@@ -222,6 +222,10 @@ struct OpSlicePassA {
     int ldh;
     float psi0;
     float factor;          // lambda * gamma
+    float* vout;           // nullable: the slice's potential rows, (B, nS, n1, N)
+    int64_t v_img_stride;  // nS n1 N
+    int64_t v_slice_off;   // slice n1 N
+    int rows_per_img;      // n1
     // raw half-spectrum values of the pair's two rows at this thread's k
     __device__ __forceinline__ void load_y(int64_t pair, int t, bool ok, float2 (&ya)[EPT],
                                            float2 (&yb)[EPT]) const {
@@ -272,6 +276,17 @@ struct OpSlicePassA {
         }
         fft_n<N, EPT, false>(w, re, im, t, tw);
     }
+    __device__ __forceinline__ void store_v(int64_t pair, int t, bool ok, const float (&va)[EPT],
+                                            const float (&vb)[EPT]) const {
+        if (!ok || !vout) return;
+        const int64_t r = pair * 2, img = r / rows_per_img;
+        float* dst = vout + img * v_img_stride + v_slice_off + (r - img * rows_per_img) * N;
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) {
+            dst[t + m * T] = va[m];
+            dst[N + t + m * T] = vb[m];
+        }
+    }
     __device__ __forceinline__ void store_w(int64_t pair, int row, int t, bool ok,
                                             const float2 (&w)[EPT]) const {
         if (!ok) return;
@@ -320,6 +335,7 @@ __global__ __launch_bounds__(RowCfg<N>::BLOCK, EMSIM_PASSA_MIN_WAVES) void k_sli
             va[m] = fmaxf(z[m].x, 1e-13f);
             vb[m] = fmaxf(z[m].y, 1e-13f);
         }
+        op.store_v(it, t, ok, va, vb);
         op.transmit_row(w0, va, t, re, im, twc);
         op.store_w(it, 0, t, ok, w0);
         if (r + 1 < rpw) {

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define EMSIM_HIP_ABI_VERSION 1
+#define EMSIM_HIP_ABI_VERSION 2
 
 #define EMSIM_HIP_OK 0
 #define EMSIM_HIP_EINVAL -1     /* bad argument (shape, size, null pointer) */
@@ -174,15 +174,111 @@ int emsim_hip_intensity(const float* wave_dev, float* out_dev, int64_t n,
  * (sqrt(dose d1 d2), wave_propagator_base.py:25-29).
  * mode 0: out_dev = (batch, n1, n2) float32 images (lens and |psi|^2 with
  * cs_mm, defocus, aperture); mode 1: out_dev = (batch, n1, n2) complex64 exit
- * waves (the lens arguments are ignored). */
+ * waves (the lens arguments are ignored).  slices_out_dev (nullable)
+ * receives the projected potential slices, (batch, n_slices, n1, n2)
+ * float32, as make_slices would return them. */
 size_t emsim_hip_simulate_workspace_size(const emsim_hip_slice_builder* b, int batch,
                                          int64_t n_atoms, int mode);
 int emsim_hip_simulate(const emsim_hip_slice_builder* b, const emsim_hip_wave_propagator* w,
                        const float* coords_dev, int64_t n_atoms, const int64_t* mol_off_host,
                        const int64_t* elem_counts_host, int batch, float psi0, float cs_mm,
                        float defocus, float aperture, int symmetric_bandlimit, float* out_dev,
-                       int mode, void* workspace_dev, size_t workspace_bytes,
-                       emsim_stream_t stream);
+                       float* slices_out_dev, int mode, void* workspace_dev,
+                       size_t workspace_bytes, emsim_stream_t stream);
+
+/* ---- learned coarse-slice correction ------------------------------------
+ * No counterpart in the reference (emsim has no learned component).  A
+ * Fourier neural operator (lifting, `layers` spectral layers of `channels`
+ * channels keeping `modes` x-frequencies on each side and `modes`
+ * y-frequencies, GELU between layers, linear projection to 2 channels) maps
+ * a coarse-slice exit wave plus its projected potentials to the correction
+ * towards the fine-slice exit wave.  Activations are channels-last bf16;
+ * every dense contraction runs on bf16 MFMA with float32 accumulation. */
+
+#define EMSIM_DT_F32 0
+#define EMSIM_DT_BF16 1
+
+/* One batched, strided GEMM, C = A B (+ A2 B2) (+ bias, GELU):
+ * element (b, i, k) of A is at a + b a_bs + i a_sm + k a_sk (in elements of
+ * a_type), likewise B (b, k, j), C (b, i, j).  k2 > 0 adds a second operand
+ * pair summed over k2.  split_k > 0: the k range is cut into chunks of
+ * split_k and chunk s writes its float32 partial sum to c + s c_split_stride
+ * (batch must be 1; no bias / activation).  beta != 0 adds beta * C_old.
+ * z (nullable) receives the pre-activation in C's layout.  Operands are
+ * rounded to bf16 on the way into the MFMA. */
+typedef struct emsim_gemm_desc {
+    int64_t m, n, k, batch;
+    const void* a;
+    int64_t a_bs, a_sm, a_sk;
+    int32_t a_type;
+    const void* b;
+    int64_t b_bs, b_sk, b_sn;
+    int32_t b_type;
+    int64_t k2;
+    const void* a2;
+    int64_t a2_bs, a2_sm, a2_sk;
+    int32_t a2_type;
+    const void* b2;
+    int64_t b2_bs, b2_sk, b2_sn;
+    int32_t b2_type;
+    void* c;
+    int64_t c_bs, c_sm, c_sn;
+    int32_t c_type;
+    const float* bias;
+    int32_t act; /* 0 identity, 1 GELU (erf form) */
+    void* z;
+    int32_t z_type;
+    float beta;
+    int64_t split_k, c_split_stride;
+} emsim_gemm_desc;
+int emsim_hip_gemm(const emsim_gemm_desc* desc, emsim_stream_t stream);
+
+typedef struct emsim_hip_fno emsim_hip_fno;
+/* n: lateral size (power of two, 16..4096); in_channels = 2 + number of
+ * potential slices; channels 16, 32 or 64; 1 <= modes <= n / 4; 1..8 layers.
+ * Parameters start at zero; load them with emsim_hip_fno_set_params. */
+int emsim_hip_fno_create(emsim_hip_fno** out, int n, int in_channels, int channels, int modes,
+                         int layers);
+void emsim_hip_fno_destroy(emsim_hip_fno* f);
+/* Number of float32 parameters, and the offsets of the parameter blocks:
+ * lift_w (cin_pad, C), lift_b (C), per layer {w (C, C), b (C),
+ * r (2 modes, modes, C, C) complex}, proj_w (C, 2), proj_b (2);
+ * cin_pad = in_channels rounded up to 16.  cap >= 4 + 3 layers. */
+int64_t emsim_hip_fno_param_count(const emsim_hip_fno* f);
+int emsim_hip_fno_param_offsets(const emsim_hip_fno* f, int64_t* offsets_host, int cap);
+int emsim_hip_fno_set_params(emsim_hip_fno* f, const float* params_dev, emsim_stream_t stream);
+int emsim_hip_fno_get_params(const emsim_hip_fno* f, float* params_dev, emsim_stream_t stream);
+int emsim_hip_fno_get_grads(const emsim_hip_fno* f, float* grads_dev, emsim_stream_t stream);
+size_t emsim_hip_fno_workspace_size(const emsim_hip_fno* f, int batch, int training);
+/* x_dev (batch, n, n, cin_pad) bf16 = [Re psi / psi0, Im psi / psi0,
+ * sigma V_1 .. sigma V_ns, 0...] from wave_dev (batch, n, n) complex64 and
+ * slices_dev (batch, ns, n, n) float32. */
+int emsim_hip_fno_pack_input(const emsim_hip_fno* f, const float* wave_dev,
+                             const float* slices_dev, int n_slices, float psi0, float sigma,
+                             int batch, void* x_dev, emsim_stream_t stream);
+/* out_dev (batch, n, n, 2) float32.  training != 0 keeps what the backward
+ * pass needs in the workspace (sized with training = 1). */
+int emsim_hip_fno_forward(emsim_hip_fno* f, const void* x_dev, float* out_dev, int batch,
+                          int training, void* workspace_dev, size_t workspace_bytes,
+                          emsim_stream_t stream);
+/* Gradients of the parameters (overwritten) for dout_dev = dL / d out, after
+ * a training forward pass with the same x_dev and workspace. */
+int emsim_hip_fno_backward(emsim_hip_fno* f, const void* x_dev, const float* dout_dev, int batch,
+                           void* workspace_dev, size_t workspace_bytes, emsim_stream_t stream);
+int emsim_hip_fno_adam(emsim_hip_fno* f, float lr, float beta1, float beta2, float eps,
+                       float weight_decay, int step, emsim_stream_t stream);
+/* wave_dev = coarse_wave_dev + psi0 (out_r + i out_i), complex64. */
+int emsim_hip_fno_apply(const emsim_hip_fno* f, const float* coarse_wave_dev, const float* out_dev,
+                        float psi0, float* wave_dev, int batch, emsim_stream_t stream);
+/* loss_dev (batch, 3): |pred - target|^2 / |target|^2, |target|^2 and
+ * |pred|^2 per image (complex64 waves of npix pixels); dout_dev (nullable,
+ * batch npix 2):
+ * the gradient of the batch-mean loss with respect to the network output
+ * when pred = coarse + psi0 out. */
+size_t emsim_hip_fno_loss_workspace_size(int64_t npix, int batch);
+int emsim_hip_fno_loss(const float* pred_dev, const float* target_dev, int64_t npix, int batch,
+                       float psi0, float* loss_dev, float* dout_dev, void* workspace_dev,
+                       size_t workspace_bytes, emsim_stream_t stream);
 
 /* ---- instrumentation (no counterpart in the reference, which has no
  * profiler; simulator.py:77,89-91 only logs wall time per image) ---------- */

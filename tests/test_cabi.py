@@ -52,7 +52,7 @@ def test_argument_errors_without_a_device():
     assert rc == hip_ext.EINVAL
     assert hip_ext.raw("emsim_hip_intensity")(None, None, 10, None) == hip_ext.EINVAL
     assert hip_ext.fft_size_supported(512) and not hip_ext.fft_size_supported(500)
-    assert hip_ext.abi_version() == 1
+    assert hip_ext.abi_version() == 2
 
 
 def test_backend_registry():

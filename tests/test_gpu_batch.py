@@ -98,3 +98,18 @@ def test_batched_equals_single_and_empty_molecule():
     # no atoms: V = 1e-13 everywhere (the clip), so the image is the lens
     # acting on an (essentially) plane wave: uniform dose d1 d2
     assert abs(together[1].mean() - 20.0) / 20.0 < 1e-4
+
+
+def test_fused_slices_output_matches_slice_builder():
+    """The potentials the fused path can write on the way equal the
+    reference-shaped builder's, and writing them leaves the waves unchanged."""
+    tem = _tem()
+    mols = _mols([4000, 7000], 61)
+    pipe = BatchPipe(tem, 1.0, 8.0, 128, 6, ELEMS)
+    batch = pipe.upload([pipe.prepare(m) for m in mols])
+    waves, slices = pipe.exit_waves_and_slices(batch)
+    ref = pipe.build_slices(batch)
+    err = orc.rel_l2(slices.cpu().numpy(), ref.cpu().numpy())
+    print(f"fused slices vs builder: {err:.2e}")
+    assert err < 2e-6
+    assert torch.equal(waves, pipe.exit_waves(batch))
