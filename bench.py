@@ -46,7 +46,93 @@ def parse():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 4 device-to-device copies of MIB MiB (a known-size "
                          "reference for HBM byte counters) and exit")
+    ap.add_argument("--learned", default=os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                      "emsim_amd", "data", "fno_512.npz"),
+                    help="trained correction network for the learned coarse-slice line")
+    ap.add_argument("--no-learned", action="store_true")
     return ap.parse_args()
+
+
+def fno_flops_per_image(n: int, cin_pad: int, C: int, modes: int, layers: int) -> float:
+    """2 M N K over the network's GEMMs for one n x n image (truncated DFTs
+    counted as GEMMs against their bases; the per-mode channel mix adds
+    8 (2 modes) modes C^2)."""
+    M1, M2 = 2 * modes, modes
+    layer = (2.0 * (2 * M2) * C * n * n            # y-transform, per row
+             + 2.0 * (2 * M1) * (2 * M2 * C) * n   # x-transform
+             + 8.0 * M1 * M2 * C * C               # channel mix
+             + 2.0 * (2 * n) * (M2 * C) * (2 * M1) # inverse x-transform
+             + 2.0 * n * n * C * (C + 2 * M2))     # W v + inverse y-transform
+    return 2.0 * n * n * cin_pad * C + layers * layer + 2.0 * n * n * C * 2
+
+
+def learned_bench(args, cfg, tem, mols, plain_ips):
+    """Coarse (8 A) fused multislice + FNO correction + lens, gated, on the
+    same molecules; error against plain fine-slice multislice."""
+    import torch
+    from emsim_amd.backend import hip_ext
+    from emsim_amd.learned import FNO, CoarseFinePairs, LearnedPipe, wave_loss
+
+    fno = FNO.load(args.learned)
+    coarse_dz = 8.0
+    coarse_slices = fno.cin - 2
+    pairs = CoarseFinePairs(tem, n=cfg["n"], pixel=cfg["pixel"], fine_dz=cfg["dz"],
+                            fine_slices=cfg["slices"], coarse_dz=coarse_dz,
+                            coarse_slices=coarse_slices, elements=cfg["elements"])
+    lp = LearnedPipe(fno, pairs)
+    batch = pairs.fine.upload([pairs.fine.prepare(m) for m in mols])
+    for _ in range(args.warmup):
+        lp.run(batch)
+    torch.cuda.synchronize()
+    hip_ext.timing_collect()
+    hip_ext.timing_enable(True)
+    reports = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        img, rep = lp.run(batch)
+        reports.append(rep)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    hip_ext.timing_enable(False)
+    kernels = hip_ext.timing_collect()
+    assert torch.isfinite(img).all().item()
+    value = args.batch * args.steps / elapsed
+    # accuracy on the benchmark batch (outside the timed region)
+    wave, _ = lp.exit_waves(batch, check=False)
+    cw, _, fine = pairs.make(batch)
+    err = wave_loss(wave, fine)[0][:, 0].sqrt().cpu().numpy()
+    base = wave_loss(cw, fine)[0][:, 0].sqrt().cpu().numpy()
+    gemm_ms = sum(k["total_ms"] for k in kernels if k["name"].startswith("k_gemm")) / args.steps
+    mix_ms = sum(k["total_ms"] for k in kernels if k["name"].startswith("k_fno")) / args.steps
+    flops = fno_flops_per_image(cfg["n"], fno.cin_pad, fno.C, fno.modes, fno.L) * args.batch
+    tflops = flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else None
+    gpu_total = sum(k["total_ms"] for k in kernels)
+    top = sorted(kernels, key=lambda r: -r["total_ms"])[:8]
+    return {
+        "images_per_s": round(value, 2),
+        "speedup_vs_plain": round(value / plain_ips, 3),
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "slices_per_s_fine_equivalent": round(value * cfg["slices"], 1),
+        "coarse_dz": coarse_dz, "coarse_slices": coarse_slices,
+        "network": {"layers": fno.L, "channels": fno.C, "modes": fno.modes, "dtype": "bf16 MFMA, f32 accumulate",
+                    "train_steps": fno.step},
+        "exit_wave_rel_l2_vs_plain": {"mean": float(err.mean()), "max": float(err.max()),
+                                      "coarse_alone_mean": float(base.mean()),
+                                      "coarse_alone_max": float(base.max())},
+        "gate": {"checks": "norm conservation per image (1%) + one sampled image per batch "
+                           "re-simulated with plain 1 A multislice (1% rel L2)",
+                 "fallbacks": sum(1 for r in reports if r.fallback),
+                 "sampled_rel_l2": [round(r.sampled_rel_l2, 6) for r in reports]},
+        "fno": {"gflop_per_image": round(flops / args.batch / 1e9, 3),
+                "gemm_ms_per_step": round(gemm_ms, 3), "other_fno_ms_per_step": round(mix_ms, 3),
+                "achieved_tflops": None if tflops is None else round(tflops, 1),
+                "mfma_peak_tflops": 2500.0,
+                "mfma_frac": None if tflops is None else round(tflops / 2500.0, 4)},
+        "gpu_busy_frac": round(gpu_total * 1e-3 / elapsed, 4),
+        "kernels": [{"kernel": k["name"], "launches": k["count"],
+                     "mean_us": round(k["mean_ms"] * 1e3, 2),
+                     "ms_per_step": round(k["total_ms"] / args.steps, 3)} for k in top],
+    }
 
 
 def calibrate_copy(mib: int):
@@ -247,6 +333,14 @@ def main():
                         "traffic": measured_traffic(top["kernel"], args.traffic),
                         "kernel": top["kernel"], "mean_us": top["mean_us"]}
 
+    learned = None
+    if not args.no_learned and os.path.exists(args.learned) and args.n == 512 and \
+            args.slices == 100 and not args.unfused:
+        del images
+        pipe = None
+        torch.cuda.empty_cache()
+        learned = learned_bench(args, cfg, tem, mols, value)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(mols, cfg, tem)
@@ -273,6 +367,7 @@ def main():
         "gpu_busy_frac": round(gpu_total * 1e-3 / elapsed, 4) if kernels else None,
         "roofline": roofline,
         "kernels": ktable,
+        "learned": learned,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

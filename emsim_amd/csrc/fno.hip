@@ -20,6 +20,9 @@ namespace {
 
 constexpr double kTwoPi = 6.283185307179586476925286766559;
 
+// emsim_hip_fno_force_generic: run every forward pass through k_gemm only
+bool g_fno_force_generic = false;
+
 uint16_t host_bf16(float x) {
     uint32_t u;
     std::memcpy(&u, &x, 4);
@@ -343,6 +346,123 @@ int emsim_hip_fno_pack_input(const emsim_hip_fno* f, const float* wave_dev,
     return EMSIM_HIP_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// The 32-channel network with at most 16 y-modes on rows of at most 512
+// pixels runs its row-local work in k_fno_rows; anything else in k_gemm.
+bool fused_rows_ok(const emsim_hip_fno* f) {
+    return f->C == 32 && f->n <= 512 && f->n % 32 == 0 && (2 * f->M2 == 16 || 2 * f->M2 == 32) &&
+           f->cin_pad <= 32;
+}
+
+// x-transform of T, per-mode mix, inverse x-transform -> U (one layer)
+int spectral_middle(const emsim_hip_fno* f, const FnoWs& w, int batch, int l, float* pq,
+                    hipStream_t s) {
+    const int n = f->n, C = f->C, M1 = f->M1, M2 = f->M2;
+    int rc;
+    emsim_gemm_desc d = gemm_desc();
+    d.m = 2 * M1; d.n = 2 * M2 * C; d.k = n; d.batch = batch;
+    d.a = f->fx2; d.a_type = EMSIM_DT_BF16; d.a_sm = n; d.a_sk = 1;
+    d.b = w.ty; d.b_type = EMSIM_DT_F32; d.b_bs = (int64_t)n * 2 * M2 * C; d.b_sk = 2 * M2 * C; d.b_sn = 1;
+    d.c = pq; d.c_type = EMSIM_DT_F32; d.c_bs = (int64_t)2 * M1 * 2 * M2 * C; d.c_sm = 2 * M2 * C; d.c_sn = 1;
+    if ((rc = launch_gemm(d, s))) return rc;
+    const float2* R = (const float2*)(f->params + f->off_r[l]);
+    if (C == 32)
+        EMSIM_LAUNCH(0, k_fno_mix<32>, dim3(M1 * M2), dim3(256), 0, s, (const float*)pq, R, w.ob, batch, M1, M2);
+    else if (C == 16)
+        EMSIM_LAUNCH(0, k_fno_mix<16>, dim3(M1 * M2), dim3(256), 0, s, (const float*)pq, R, w.ob, batch, M1, M2);
+    else
+        EMSIM_LAUNCH(0, k_fno_mix<64>, dim3(M1 * M2), dim3(256), 0, s, (const float*)pq, R, w.ob, batch, M1, M2);
+    d = gemm_desc();
+    d.m = 2 * n; d.n = M2 * C; d.k = 2 * M1; d.batch = batch;
+    d.a = f->gx; d.a_type = EMSIM_DT_BF16; d.a_sm = 2 * M1; d.a_sk = 1;
+    d.b = w.ob; d.b_type = EMSIM_DT_F32; d.b_bs = (int64_t)2 * M1 * M2 * C; d.b_sk = M2 * C; d.b_sn = 1;
+    d.c = w.u; d.c_type = EMSIM_DT_F32; d.c_bs = (int64_t)2 * n * M2 * C; d.c_sm = M2 * C; d.c_sn = 1;
+    return launch_gemm(d, s);
+}
+
+// the row-kernel modes the forward pass uses
+constexpr int kModeLift = kRowV | kRowT;
+constexpr int kModeMid = kRowU | kRowAct | kRowV | kRowT;
+constexpr int kModeMidTrain = kModeMid | kRowZ;
+constexpr int kModeLast = kRowU | kRowProj;
+constexpr int kModeLastTrain = kModeLast | kRowV | kRowZ;
+
+template <int K1F, int K2F>
+int launch_rows_mode(const FnoRowArgs& a, int mode, dim3 grid, size_t lds, hipStream_t s) {
+    const dim3 block(256);
+    switch (mode) {
+    case kModeLift: EMSIM_LAUNCH(0, (k_fno_rows<K1F, K2F, kModeLift>), grid, block, lds, s, a); break;
+    case kModeMid: EMSIM_LAUNCH(0, (k_fno_rows<K1F, K2F, kModeMid>), grid, block, lds, s, a); break;
+    case kModeMidTrain: EMSIM_LAUNCH(0, (k_fno_rows<K1F, K2F, kModeMidTrain>), grid, block, lds, s, a); break;
+    case kModeLast: EMSIM_LAUNCH(0, (k_fno_rows<K1F, K2F, kModeLast>), grid, block, lds, s, a); break;
+    case kModeLastTrain: EMSIM_LAUNCH(0, (k_fno_rows<K1F, K2F, kModeLastTrain>), grid, block, lds, s, a); break;
+    default: return fail(EMSIM_HIP_EINVAL, "fno rows: unsupported mode %d", mode);
+    }
+    return EMSIM_HIP_OK;
+}
+
+int launch_rows(const FnoRowArgs& a, int mode, int64_t rows, hipStream_t s) {
+    const dim3 grid((unsigned)rows);
+    const size_t lds = kFnoRowLds(a.n);
+    if (a.k1 == 16) {
+        if (mode != kModeLift) return fail(EMSIM_HIP_EINVAL, "fno rows: 16 input channels only lift");
+        if (a.m2x2 == 16)
+            EMSIM_LAUNCH(0, (k_fno_rows<1, 1, kModeLift>), grid, dim3(256), lds, s, a);
+        else
+            EMSIM_LAUNCH(0, (k_fno_rows<1, 2, kModeLift>), grid, dim3(256), lds, s, a);
+        return EMSIM_HIP_OK;
+    }
+    return a.m2x2 == 16 ? launch_rows_mode<2, 1>(a, mode, grid, lds, s)
+                        : launch_rows_mode<2, 2>(a, mode, grid, lds, s);
+}
+
+int forward_fused(const emsim_hip_fno* f, const FnoWs& w, const void* x, float* out, int batch,
+                  int training, hipStream_t s) {
+    const int64_t rows = (int64_t)batch * f->n;
+    FnoRowArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.n = f->n;
+    a.gy = f->gy;
+    a.m2x2 = 2 * f->M2;
+    a.T = w.ty;
+    // lifting, and the y-transform of its output
+    a.vin = (const uint16_t*)x;
+    a.k1 = f->cin_pad;
+    a.w16 = f->params16 + f->off_lw;
+    a.bias = f->params + f->off_lb;
+    a.vout = w.v[0];
+    a.fy = f->fy;
+    int rc = launch_rows(a, kModeLift, rows, s);
+    if (rc) return rc;
+    for (int l = 0; l < f->L; ++l) {
+        const bool last = l + 1 == f->L;
+        if ((rc = spectral_middle(f, w, batch, l, training ? w.pq[l] : w.pq[0], s))) return rc;
+        a.vin = training ? w.v[l] : w.v[l & 1];
+        a.k1 = f->C;
+        a.w16 = f->params16 + f->off_w[l];
+        a.bias = f->params + f->off_b[l];
+        a.U = w.u;
+        a.act = last ? 0 : 1;
+        a.vout = training ? w.v[l + 1] : (last ? nullptr : w.v[(l + 1) & 1]);
+        a.zout = training ? w.z[l] : nullptr;
+        a.fy = last ? nullptr : f->fy;
+        a.q16 = last ? f->params16 + f->off_qw : nullptr;
+        a.qb = f->params + f->off_qb;
+        a.out = out;
+        const int mode = last ? (training ? kModeLastTrain : kModeLast)
+                              : (training ? kModeMidTrain : kModeMid);
+        if ((rc = launch_rows(a, mode, rows, s))) return rc;
+    }
+    return EMSIM_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int emsim_hip_fno_forward(emsim_hip_fno* f, const void* x_dev, float* out_dev, int batch,
                           int training, void* workspace_dev, size_t workspace_bytes,
                           emsim_stream_t stream) {
@@ -353,6 +473,8 @@ int emsim_hip_fno_forward(emsim_hip_fno* f, const void* x_dev, float* out_dev, i
     if (workspace_bytes < w.bytes)
         return fail(EMSIM_HIP_EWORKSPACE, "fno_forward: workspace %zu < %zu", workspace_bytes, w.bytes);
     hipStream_t s = (hipStream_t)stream;
+    if (fused_rows_ok(f) && !g_fno_force_generic)
+        return forward_fused(f, w, x_dev, out_dev, batch, training, s);
     const int n = f->n, C = f->C, M1 = f->M1, M2 = f->M2;
     const int64_t P = (int64_t)batch * n * n;
     int rc;
@@ -377,28 +499,8 @@ int emsim_hip_fno_forward(emsim_hip_fno* f, const void* x_dev, float* out_dev, i
         d.b = vin; d.b_type = EMSIM_DT_BF16; d.b_bs = (int64_t)n * C; d.b_sk = C; d.b_sn = 1;
         d.c = w.ty; d.c_type = EMSIM_DT_F32; d.c_bs = 2 * M2 * C; d.c_sm = C; d.c_sn = 1;
         if ((rc = launch_gemm(d, s))) return rc;
-        // P, Q = sum_x (cos, sin)(k1 x) T   per image
-        d = gemm_desc();
-        d.m = 2 * M1; d.n = 2 * M2 * C; d.k = n; d.batch = batch;
-        d.a = f->fx2; d.a_type = EMSIM_DT_BF16; d.a_sm = n; d.a_sk = 1;
-        d.b = w.ty; d.b_type = EMSIM_DT_F32; d.b_bs = (int64_t)n * 2 * M2 * C; d.b_sk = 2 * M2 * C; d.b_sn = 1;
-        d.c = pq; d.c_type = EMSIM_DT_F32; d.c_bs = (int64_t)2 * M1 * 2 * M2 * C; d.c_sm = 2 * M2 * C; d.c_sn = 1;
-        if ((rc = launch_gemm(d, s))) return rc;
-        // per-mode complex channel mix
-        const float2* R = (const float2*)(f->params + f->off_r[l]);
-        if (C == 32)
-            EMSIM_LAUNCH(0, k_fno_mix<32>, dim3(M1 * M2), dim3(256), 0, s, (const float*)pq, R, w.ob, batch, M1, M2);
-        else if (C == 16)
-            EMSIM_LAUNCH(0, k_fno_mix<16>, dim3(M1 * M2), dim3(256), 0, s, (const float*)pq, R, w.ob, batch, M1, M2);
-        else
-            EMSIM_LAUNCH(0, k_fno_mix<64>, dim3(M1 * M2), dim3(256), 0, s, (const float*)pq, R, w.ob, batch, M1, M2);
-        // U = sum_k1 exp(+i k1 x) O          per image
-        d = gemm_desc();
-        d.m = 2 * n; d.n = M2 * C; d.k = 2 * M1; d.batch = batch;
-        d.a = f->gx; d.a_type = EMSIM_DT_BF16; d.a_sm = 2 * M1; d.a_sk = 1;
-        d.b = w.ob; d.b_type = EMSIM_DT_F32; d.b_bs = (int64_t)2 * M1 * M2 * C; d.b_sk = M2 * C; d.b_sn = 1;
-        d.c = w.u; d.c_type = EMSIM_DT_F32; d.c_bs = (int64_t)2 * n * M2 * C; d.c_sm = M2 * C; d.c_sn = 1;
-        if ((rc = launch_gemm(d, s))) return rc;
+        // P, Q = sum_x (cos, sin)(k1 x) T; per-mode mix; U = sum_k1 exp(+i k1 x) O
+        if ((rc = spectral_middle(f, w, batch, l, pq, s))) return rc;
         // v' = act(v W + b + irfft_y(U))    per image row
         d = gemm_desc();
         d.m = n; d.n = C; d.k = C; d.batch = (int64_t)batch * n;
@@ -515,6 +617,11 @@ int emsim_hip_fno_backward(emsim_hip_fno* f, const void* x_dev, const float* dou
                           C, w.dout_partial, g + f->off_lw, s)))
         return rc;
     return colsum(w.dv, P, C, w.colsum_part, g + f->off_lb, s);
+}
+
+int emsim_hip_fno_force_generic(int on) {
+    g_fno_force_generic = on != 0;
+    return EMSIM_HIP_OK;
 }
 
 int emsim_hip_fno_adam(emsim_hip_fno* f, float lr, float beta1, float beta2, float eps,

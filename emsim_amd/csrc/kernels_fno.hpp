@@ -22,12 +22,16 @@ namespace emsim {
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
-// float32 -> bf16, round to nearest even (NaN stays NaN)
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+
+// float32 -> bf16, round to nearest even (one v_cvt_pk_bf16_f32 per pair)
 __device__ __forceinline__ uint16_t f32_to_bf16(float x) {
-    uint32_t u = __float_as_uint(x);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
+    return __builtin_bit_cast(uint16_t, (__bf16)x);
+}
+__device__ __forceinline__ uint32_t f32x2_to_bf16x2(float lo, float hi) {
+    const f32x2_t v = {lo, hi};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2_t));
 }
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
@@ -35,10 +39,22 @@ __device__ __forceinline__ float ld_elem(const void* p, int64_t off, int type) {
     return type == EMSIM_DT_BF16 ? bf16_to_f32(((const uint16_t*)p)[off]) : ((const float*)p)[off];
 }
 
+// erf by Abramowitz & Stegun 7.1.26 (after Hastings): branch-free, absolute
+// error <= 1.5e-7, far below the bf16 rounding of the activations it feeds.
+__device__ __forceinline__ float erf_as(float x) {
+    const float ax = fabsf(x);
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, ax, 1.f));
+    float p = fmaf(1.061405429f, t, -1.453152027f);
+    p = fmaf(p, t, 1.421413741f);
+    p = fmaf(p, t, -0.284496736f);
+    p = fmaf(p, t, 0.254829592f);
+    return copysignf(1.f - p * t * __expf(-ax * ax), x);
+}
+
 // GELU with the error function, and its derivative
-__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.f + erff(x * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.f + erf_as(x * 0.70710678118654752f)); }
 __device__ __forceinline__ float gelu_grad(float x) {
-    return 0.5f * (1.f + erff(x * 0.70710678118654752f)) +
+    return 0.5f * (1.f + erf_as(x * 0.70710678118654752f)) +
            x * 0.39894228040143268f * __expf(-0.5f * x * x);
 }
 
@@ -147,6 +163,194 @@ __global__ __launch_bounds__(kGemmThreads) void k_gemm(emsim_gemm_desc d) {
         }
         if (d.c_type == EMSIM_DT_BF16) ((uint16_t*)d.c)[off] = f32_to_bf16(val);
         else ((float*)d.c)[off] = val;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused row kernel of the 32-channel network: one block per image row
+// (b, x), four waves, wave w owning the 32-pixel tiles y0 = 32 (w + 4 i).
+// Channels are the MFMA rows and pixels its columns, so every operand
+// fragment is 16 contiguous bytes (8 bf16 channels of one pixel, or 8
+// pixels of one channel in LDS):
+//   z^T = W^T v^T + Uhat^T Gy^T + b      (lifting: Lw^T X^T + b, no Uhat)
+//   v'  = act(z)  -> vout (bf16), z -> zout (training), v'^T -> LDS
+//   then either T^T = v'^T Fy^T, the next layer's y-transform of this row
+//   (float32, (2 M2, C) per row), or the projection out = v' Qw + bq.
+// Needs C = 32, k1 = 16 or 32, 2 M2 = 16 or 32, n a multiple of 32 and at
+// most 512 (four 32-pixel tiles per wave).  Dynamic LDS: kFnoRowLds(n) bytes.
+// ---------------------------------------------------------------------------
+struct FnoRowArgs {
+    const uint16_t* vin;   // (rows, n, k1) bf16
+    int k1;                // input channels (C, or cin_pad for the lifting)
+    const uint16_t* w16;   // (k1, 32) bf16, [k][c]
+    const float* bias;     // (32)
+    const float* U;        // (rows, 2 M2, 32) float32: Uhat of every row, or null
+    const uint16_t* gy;    // (n, 2 M2) bf16
+    int m2x2;              // 2 M2
+    int act;               // GELU after the sum
+    uint16_t* vout;        // (rows, n, 32) bf16, or null
+    uint16_t* zout;        // (rows, n, 32) bf16 pre-activation, or null
+    const uint16_t* fy;    // (2 M2, n) bf16, or null (no y-transform of the output)
+    float* T;              // (rows, 2 M2, 32) float32
+    const uint16_t* q16;   // (32, 2) bf16 projection, or null
+    const float* qb;       // (2)
+    float* out;            // (rows, n, 2) float32
+    int n;
+};
+
+constexpr int kFnoWStride = 72;    // bf16 per LDS row of W^T / Uhat^T (144 B, 16-B aligned)
+__host__ __device__ constexpr int fno_vt_stride(int n) { return n + 8; }
+// W^T and Uhat^T, then v'^T -- reused afterwards for the four waves'
+// partial sums of T (4 x 32 x 33 float32), so it is at least that large
+__host__ __device__ constexpr size_t kFnoRowLds(int n) {
+    return (size_t)2 * 32 * kFnoWStride * 2 +
+           ((size_t)32 * fno_vt_stride(n) * 2 > (size_t)4 * 32 * 33 * 4
+                ? (size_t)32 * fno_vt_stride(n) * 2
+                : (size_t)4 * 32 * 33 * 4);
+}
+
+// K1F = k1 / 16 and K2F = 2 M2 / 16 fragments per 32-pixel tile (1 or 2).
+// Every global load of a wave (its tiles' input channels, Gy rows and Fy
+// columns) is issued before the block's first barrier, so their latency
+// overlaps the staging of W^T and Uhat^T.
+// MODE bits: the sum has the Uhat term, GELU, v' written, z written, the
+// next y-transform computed, the projection computed.
+constexpr int kRowU = 1, kRowAct = 2, kRowV = 4, kRowZ = 8, kRowT = 16, kRowProj = 32;
+
+template <int K1F, int K2F, int MODE>
+__global__ __launch_bounds__(256) void k_fno_rows(FnoRowArgs a) {
+    constexpr bool HAS_U = MODE & kRowU, ACT = MODE & kRowAct, VOUT = MODE & kRowV,
+                   ZOUT = MODE & kRowZ, TOUT = MODE & kRowT, PROJ = MODE & kRowProj;
+    extern __shared__ __align__(16) uint16_t fno_lds[];
+    uint16_t* wT = fno_lds;                         // [c][k]
+    uint16_t* uT = wT + 32 * kFnoWStride;           // [c][j]
+    uint16_t* vT = uT + 32 * kFnoWStride;           // [c][y]
+    constexpr int k1 = 16 * K1F, m2x2 = 16 * K2F;
+    const int n = a.n, vts = fno_vt_stride(n);
+    const int64_t row = blockIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int l32 = lane & 31, h = lane >> 5;
+    const uint16_t* vrow = a.vin + row * n * k1;
+
+    bf16x8_t vf[4][K1F], gf[4][K2F], ff[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int y0 = 32 * (w + 4 * i), y = y0 + l32;
+        const bool ok = y0 < n;
+#pragma unroll
+        for (int f = 0; f < K1F; ++f) {
+            vf[i][f] = bf16x8_t{};
+            if (ok) vf[i][f] = *(const bf16x8_t*)(vrow + (int64_t)y * k1 + 16 * f + 8 * h);
+        }
+#pragma unroll
+        for (int f = 0; f < K2F; ++f) {
+            gf[i][f] = bf16x8_t{};
+            if (HAS_U && ok) gf[i][f] = *(const bf16x8_t*)(a.gy + (int64_t)y * m2x2 + 16 * f + 8 * h);
+        }
+#pragma unroll
+        for (int f = 0; f < 2; ++f) {
+            ff[i][f] = bf16x8_t{};
+            if (TOUT && ok && l32 < m2x2)
+                ff[i][f] = *(const bf16x8_t*)(a.fy + (int64_t)l32 * n + y0 + 16 * f + 8 * h);
+        }
+    }
+
+    __shared__ float lds_bias[32], lds_q[64];
+    for (int i = threadIdx.x; i < k1 * 32; i += 256) {
+        const int k = i >> 5, c = i & 31;
+        wT[c * kFnoWStride + k] = a.w16[i];
+    }
+    if (HAS_U) {
+        const float* u = a.U + row * m2x2 * 32;
+        for (int i = threadIdx.x; i < m2x2 * 32; i += 256) {
+            const int j = i >> 5, c = i & 31;
+            uT[c * kFnoWStride + j] = f32_to_bf16(u[i]);
+        }
+    }
+    if (threadIdx.x < 32) lds_bias[threadIdx.x] = a.bias[threadIdx.x];
+    if (PROJ && threadIdx.x < 64) lds_q[threadIdx.x] = bf16_to_f32(a.q16[threadIdx.x]);
+    __syncthreads();
+
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int y0 = 32 * (w + 4 * i), y = y0 + l32;
+        if (y0 >= n) break;
+        f32x16_t acc = {};
+#pragma unroll
+        for (int f = 0; f < K1F; ++f) {
+            const bf16x8_t fa = *(const bf16x8_t*)(wT + l32 * kFnoWStride + 16 * f + 8 * h);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa, vf[i][f], acc, 0, 0, 0);
+        }
+        if (HAS_U) {
+#pragma unroll
+            for (int f = 0; f < K2F; ++f) {
+                const bf16x8_t fa = *(const bf16x8_t*)(uT + l32 * kFnoWStride + 16 * f + 8 * h);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa, gf[i][f], acc, 0, 0, 0);
+            }
+        }
+        const int64_t pix = row * n + y;
+        float p0 = 0.f, p1 = 0.f;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const int c0 = 8 * g + 4 * h;
+            float z[4], v[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                z[j] = acc[4 * g + j] + lds_bias[c0 + j];
+                v[j] = ACT ? gelu(z[j]) : z[j];
+            }
+            const uint2 vb = make_uint2(f32x2_to_bf16x2(v[0], v[1]), f32x2_to_bf16x2(v[2], v[3]));
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t word = j < 2 ? vb.x : vb.y;
+                const uint16_t bits = (uint16_t)((j & 1) ? word >> 16 : word);
+                vT[(c0 + j) * vts + y] = bits;
+                if (PROJ) {
+                    const float vv = bf16_to_f32(bits);
+                    p0 += vv * lds_q[2 * (c0 + j)];
+                    p1 += vv * lds_q[2 * (c0 + j) + 1];
+                }
+            }
+            if (ZOUT)
+                *(uint2*)(a.zout + pix * 32 + c0) =
+                    make_uint2(f32x2_to_bf16x2(z[0], z[1]), f32x2_to_bf16x2(z[2], z[3]));
+            if (VOUT) *(uint2*)(a.vout + pix * 32 + c0) = vb;
+        }
+        if (PROJ) {
+            p0 += __shfl_xor(p0, 32);
+            p1 += __shfl_xor(p1, 32);
+            if (h == 0) *(float2*)(a.out + pix * 2) = make_float2(p0 + a.qb[0], p1 + a.qb[1]);
+        }
+    }
+    if (!TOUT) return;
+    __syncthreads();
+    // T^T (c, m2) = sum_y v'^T (c, y) Fy^T (y, m2), each wave over its tiles
+    f32x16_t acc = {};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int y0 = 32 * (w + 4 * i);
+        if (y0 >= n) break;
+#pragma unroll
+        for (int f = 0; f < 2; ++f) {
+            const bf16x8_t fa = *(const bf16x8_t*)(vT + l32 * vts + y0 + 16 * f + 8 * h);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa, ff[i][f], acc, 0, 0, 0);
+        }
+    }
+    __syncthreads();   // vT is reused for the partial sums of the four waves
+    float* red = (float*)vT;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const int c = 8 * (v >> 2) + 4 * h + (v & 3);
+        red[(w * 32 + c) * 33 + l32] = acc[v];
+    }
+    __syncthreads();
+    float* trow = a.T + row * m2x2 * 32;
+    for (int i = threadIdx.x; i < m2x2 * 32; i += 256) {
+        const int m = i >> 5, c = i & 31;
+        float s = 0.f;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s += red[(q * 32 + c) * 33 + m];
+        trow[i] = s;
     }
 }
 
@@ -361,12 +565,22 @@ __global__ void k_fno_pack(const float2* __restrict__ wave, const float* __restr
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
         const int64_t b = i / npix, p = i - b * npix;
         const float2 w = wave[i];
-        uint16_t* xo = x + i * cin_pad;
-        xo[0] = f32_to_bf16(w.x * inv_psi0);
-        xo[1] = f32_to_bf16(w.y * inv_psi0);
         const float* v = slices + b * ns * npix + p;
-        for (int s = 0; s < ns; ++s) xo[2 + s] = f32_to_bf16(sigma * v[(int64_t)s * npix]);
-        for (int c = 2 + ns; c < cin_pad; ++c) xo[c] = 0;
+        auto chan = [&](int c) -> float {
+            if (c == 0) return w.x * inv_psi0;
+            if (c == 1) return w.y * inv_psi0;
+            return c < 2 + ns ? sigma * v[(int64_t)(c - 2) * npix] : 0.f;
+        };
+        // cin_pad is a multiple of 16: whole 16-byte groups of 8 channels
+        uint4* xo = (uint4*)(x + i * cin_pad);
+        for (int g = 0; g < cin_pad / 8; ++g) {
+            uint32_t q[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                q[j] = f32_to_bf16(chan(8 * g + 2 * j)) |
+                       ((uint32_t)f32_to_bf16(chan(8 * g + 2 * j + 1)) << 16);
+            xo[g] = make_uint4(q[0], q[1], q[2], q[3]);
+        }
     }
 }
 

@@ -261,6 +261,10 @@ int emsim_hip_fno_pack_input(const emsim_hip_fno* f, const float* wave_dev,
 int emsim_hip_fno_forward(emsim_hip_fno* f, const void* x_dev, float* out_dev, int batch,
                           int training, void* workspace_dev, size_t workspace_bytes,
                           emsim_stream_t stream);
+/* on != 0: every forward pass uses the generic GEMM sequence instead of the
+ * fused row kernels (which serve 32 channels, <= 16 modes, n <= 512); the
+ * two compute the same network, this selects one for cross-checks. */
+int emsim_hip_fno_force_generic(int on);
 /* Gradients of the parameters (overwritten) for dout_dev = dL / d out, after
  * a training forward pass with the same x_dev and workspace. */
 int emsim_hip_fno_backward(emsim_hip_fno* f, const void* x_dev, const float* dout_dev, int batch,

@@ -192,6 +192,20 @@ def test_forward_matches_oracle(net):
     assert np.array_equal(got_t, got.astype(np.float32))
 
 
+def test_fused_rows_equal_generic_path(hip, net):
+    """The fused row kernels and the all-GEMM sequence compute the same network."""
+    f, _, xt, _ = net
+    fused = f.forward(xt).cpu().numpy()
+    hip.call("emsim_hip_fno_force_generic", 1)
+    try:
+        gen = f.forward(xt).cpu().numpy()
+    finally:
+        hip.call("emsim_hip_fno_force_generic", 0)
+    rel = np.linalg.norm(fused - gen) / np.linalg.norm(gen)
+    print(f"fused vs generic forward rel L2 {rel:.2e}")
+    assert rel < 1e-3
+
+
 def test_backward_matches_oracle(net):
     f, p, xt, x_bf = net
     rng = np.random.default_rng(9)
