@@ -67,6 +67,9 @@ _SIGNATURES = {
                                    POINTER(c_int64), c_int, c_float, c_float, c_float, c_float,
                                    c_int, c_void_p, c_void_p, c_int, c_void_p, c_size_t,
                                    c_void_p]),
+    "emsim_hip_simulate_dev": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
+                                       c_int, c_float, c_float, c_float, c_float, c_int,
+                                       c_void_p, c_void_p, c_int, c_void_p, c_size_t, c_void_p]),
     "emsim_hip_gemm": (c_int, [c_void_p, c_void_p]),
     "emsim_hip_fno_create": (c_int, [POINTER(c_void_p), c_int, c_int, c_int, c_int, c_int]),
     "emsim_hip_fno_destroy": (None, [c_void_p]),
@@ -91,6 +94,8 @@ _SIGNATURES = {
     "emsim_hip_fno_loss": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_float, c_void_p,
                                    c_void_p, c_void_p, c_size_t, c_void_p]),
     "emsim_hip_timing_enable": (c_int, [c_int]),
+    "emsim_hip_tuning_set": (c_int, [c_char_p, c_int]),
+    "emsim_hip_tuning_get": (c_int, [c_char_p]),
     "emsim_hip_timing_collect": (c_int, [c_void_p, c_size_t, POINTER(c_size_t)]),
 }
 

@@ -49,6 +49,18 @@ class PreparedBatch:
         return len(self.labels)
 
 
+class GraphRun:
+    """A captured launch sequence of the fused path (``BatchPipe.capture``)."""
+
+    def __init__(self, graph, out: torch.Tensor, coords: torch.Tensor, keep):
+        self.graph, self.out, self.coords = graph, out, coords
+        self._keep = keep              # device buffers the graph reads
+
+    def replay(self) -> torch.Tensor:
+        self.graph.replay()
+        return self.out
+
+
 class BatchPipe:
     def __init__(self, microscope: TEM, pixel_size: float, slice_thickness: float,
                  roi: int, n_slices: int, elements: Sequence[int] = (1, 6, 7, 8, 16),
@@ -123,6 +135,53 @@ class BatchPipe:
     def run(self, batch: PreparedBatch) -> torch.Tensor:
         """(B, n, n) float32 images on the GPU."""
         return self._simulate(batch, 0)
+
+    # -- captured HIP graph -----------------------------------------------------
+    def element_offsets(self, batch: PreparedBatch) -> np.ndarray:
+        """(B, nE + 1) absolute atom offsets of each molecule's element groups."""
+        B, nE = len(batch), len(self.elements)
+        counts = np.asarray(batch.counts, dtype=np.int64).reshape(B, nE)
+        if (counts < 0).any():
+            raise ValueError("negative element count")
+        off = np.empty((B, nE + 1), dtype=np.int64)
+        off[:, 0] = batch.mol_off[:-1]
+        off[:, 1:] = batch.mol_off[:-1, None] + np.cumsum(counts, axis=1)
+        if (off[:, -1] > batch.mol_off[1:]).any() or (np.diff(batch.mol_off) < 0).any():
+            raise ValueError("element counts exceed a molecule's atoms")
+        return off
+
+    def capture(self, batch: PreparedBatch, mode: int = 0) -> "GraphRun":
+        """Record the whole fused launch sequence for this batch layout in a
+        HIP graph (torch.cuda.CUDAGraph on ROCm).  Replays read the atoms from
+        ``GraphRun.coords`` (overwrite it in place with molecules of the same
+        per-element counts) and write ``GraphRun.out``.  Per-kernel timing
+        must be off while capturing."""
+        B, n_atoms = len(batch), int(batch.coords.shape[0])
+        dev = device()
+        mol_off = torch.as_tensor(batch.mol_off, dtype=torch.int64).to(dev)
+        elem_off = torch.as_tensor(self.element_offsets(batch)).to(dev)
+        coords = batch.coords.clone()
+        dtype = torch.float32 if mode == 0 else torch.complex64
+        out = torch.empty((B, self.n, self.n), dtype=dtype, device=dev)
+        ws_bytes = hip_ext.raw("emsim_hip_simulate_workspace_size")(
+            self.builder.backend.h, B, n_atoms, mode)
+        ws = torch.empty(max(int(ws_bytes), 1), dtype=torch.uint8, device=dev)
+
+        def launch():
+            mic = self.microscope
+            psi0 = self.propagator.init_wave_value(mic.electron_dose)
+            hip_ext.call("emsim_hip_simulate_dev", self.builder.backend.h,
+                         self.propagator.backend.h, ptr(coords), n_atoms, ptr(mol_off),
+                         ptr(elem_off), B, float(psi0), float(mic.cs_mm), float(mic.defocus),
+                         float(mic.aperture), 1 if self.symmetric_bandlimit else 0, ptr(out),
+                         None, mode, ptr(ws), ws.numel(), stream_handle())
+
+        launch()                       # sets up the library's per-device tables
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            launch()
+        return GraphRun(graph, out, coords, (mol_off, elem_off, ws))
 
     def exit_waves(self, batch: PreparedBatch) -> torch.Tensor:
         """(B, n, n) complex64 exit waves on the GPU."""

@@ -247,13 +247,24 @@ struct emsim_hip_wave_propagator {
 
 namespace {
 
+// Launch-shape knobs (emsim_hip_tuning_set); 0 selects the built-in rule.
+// They change how work is split over blocks, never the arithmetic.
+int g_row_items_per_block = 0;   // row groups per row-worker block
+int g_col_tiles_per_block = 0;   // column tiles per column-kernel block
+
 // Grid of the row worker for n_items rows (or row pairs) of length N.
 template <int N>
 void row_grid(int64_t n_items, int& rpw, dim3& grid) {
     using Cfg = RowCfg<N>;
     const int64_t groups = ceil_div(n_items, Cfg::G);
-    rpw = rows_per_block(groups);
+    rpw = g_row_items_per_block > 0 ? g_row_items_per_block : rows_per_block(groups);
     grid = dim3((unsigned)ceil_div(groups, rpw));
+}
+
+int64_t col_tiles_per_block(int64_t n_tiles) {
+    if (g_col_tiles_per_block > 0) return g_col_tiles_per_block;
+    const int64_t tpb = n_tiles / 512;
+    return tpb < 1 ? 1 : (tpb > 8 ? 8 : tpb);
 }
 
 template <int N>
@@ -304,8 +315,7 @@ int launch_col_mul_impl(float2* data, int ld, int ncols, int64_t img_stride, int
     using Cfg = ColCfg<N>;
     const int tiles_per_img = (int)ceil_div(ncols, Cfg::C);
     const int64_t n_tiles = (int64_t)tiles_per_img * n_img;
-    int64_t tpb = n_tiles / 512;
-    tpb = tpb < 1 ? 1 : (tpb > 8 ? 8 : tpb);
+    const int64_t tpb = col_tiles_per_block(n_tiles);
     const dim3 grid((unsigned)ceil_div(n_tiles, tpb));
     EMSIM_LAUNCH(N, (k_col_fft_mul_ifft<N, Mul>), grid, dim3(Cfg::BLOCK), 0, s, data, ld, ncols,
                  tiles_per_img, img_stride, n_tiles, (int)tpb, mul, tw);
@@ -883,6 +893,29 @@ extern "C" int emsim_hip_timing_enable(int on) {
     return EMSIM_HIP_OK;
 }
 
+// ==========================================================================
+// launch-shape knobs
+// ==========================================================================
+extern "C" int emsim_hip_tuning_set(const char* key, int value) {
+    if (!key) return fail(EMSIM_HIP_EINVAL, "tuning_set: null key");
+    if (std::strcmp(key, "row_items_per_block") == 0) {
+        if (value < 0 || value > 64) return fail(EMSIM_HIP_EINVAL, "row_items_per_block in [0, 64]");
+        g_row_items_per_block = value;
+    } else if (std::strcmp(key, "col_tiles_per_block") == 0) {
+        if (value < 0 || value > 32) return fail(EMSIM_HIP_EINVAL, "col_tiles_per_block in [0, 32]");
+        g_col_tiles_per_block = value;
+    } else {
+        return fail(EMSIM_HIP_EINVAL, "tuning_set: unknown key '%s'", key);
+    }
+    return EMSIM_HIP_OK;
+}
+
+extern "C" int emsim_hip_tuning_get(const char* key) {
+    if (key && std::strcmp(key, "row_items_per_block") == 0) return g_row_items_per_block;
+    if (key && std::strcmp(key, "col_tiles_per_block") == 0) return g_col_tiles_per_block;
+    return fail(EMSIM_HIP_EINVAL, "tuning_get: unknown key");
+}
+
 extern "C" int emsim_hip_timing_collect(char* buf, size_t cap, size_t* needed) {
     struct Agg {
         std::string name;
@@ -1048,6 +1081,33 @@ FusedWorkspace fused_layout(const emsim_hip_slice_builder* b, int batch, int64_t
     return w;
 }
 
+int check_simulate(const emsim_hip_slice_builder* b, const emsim_hip_wave_propagator* w,
+                   const float* coords, int64_t n_atoms, int batch, int symmetric_bandlimit,
+                   const float* out, int mode) {
+    if (!b || !w || !out || batch < 1 || n_atoms < 0 || (n_atoms > 0 && !coords) ||
+        (mode != 0 && mode != 1))
+        return fail(EMSIM_HIP_EINVAL, "simulate: bad arguments");
+    if (b->one_slice)
+        return fail(EMSIM_HIP_EINVAL, "simulate: needs a multi-slice builder");
+    if (w->n1 != b->n1 || w->n2 != b->n2)
+        return fail(EMSIM_HIP_EINVAL, "simulate: builder grid %dx%d != propagator grid %dx%d",
+                    b->n1, b->n2, w->n1, w->n2);
+    if (symmetric_bandlimit && b->n1 != b->n2)
+        return fail(EMSIM_HIP_EINVAL, "simulate: the symmetric band-limit needs a square grid");
+    if (batch > 65535) return fail(EMSIM_HIP_EUNSUPPORTED, "simulate: batch <= 65535");
+    if (n_atoms >= (int64_t)0x7fffffff)
+        return fail(EMSIM_HIP_EUNSUPPORTED, "simulate: at most 2^31-1 atoms per call");
+    const int64_t n_rows_total = (int64_t)batch * b->n_slices * b->n_elems * b->n1;
+    if (n_rows_total >= (int64_t)0x7fffffff)
+        return fail(EMSIM_HIP_EUNSUPPORTED, "simulate: batch * slices * elements * n1 too large");
+    return EMSIM_HIP_OK;
+}
+
+int simulate_core(const emsim_hip_slice_builder* b, const emsim_hip_wave_propagator* w,
+                  const float* coords, int64_t n_atoms, const FusedWorkspace& ws, int batch,
+                  float psi0, float cs_mm, float defocus, float aperture, int symmetric_bandlimit,
+                  float* out, float* slices_out, int mode, hipStream_t s);
+
 }  // namespace
 
 extern "C" size_t emsim_hip_simulate_workspace_size(const emsim_hip_slice_builder* b, int batch,
@@ -1065,22 +1125,9 @@ extern "C" int emsim_hip_simulate(const emsim_hip_slice_builder* b,
                                   int mode, void* workspace, size_t workspace_bytes,
                                   emsim_stream_t stream_) {
     hipStream_t s = (hipStream_t)stream_;
-    if (!b || !w || !mol_off_host || !elem_counts_host || !out || batch < 1 || n_atoms < 0 ||
-        (n_atoms > 0 && !coords) || (mode != 0 && mode != 1))
-        return fail(EMSIM_HIP_EINVAL, "simulate: bad arguments");
-    if (b->one_slice)
-        return fail(EMSIM_HIP_EINVAL, "simulate: needs a multi-slice builder");
-    if (w->n1 != b->n1 || w->n2 != b->n2)
-        return fail(EMSIM_HIP_EINVAL, "simulate: builder grid %dx%d != propagator grid %dx%d",
-                    b->n1, b->n2, w->n1, w->n2);
-    if (symmetric_bandlimit && b->n1 != b->n2)
-        return fail(EMSIM_HIP_EINVAL, "simulate: the symmetric band-limit needs a square grid");
-    if (batch > 65535) return fail(EMSIM_HIP_EUNSUPPORTED, "simulate: batch <= 65535");
-    if (n_atoms >= (int64_t)0x7fffffff)
-        return fail(EMSIM_HIP_EUNSUPPORTED, "simulate: at most 2^31-1 atoms per call");
-    const int64_t n_rows_total = (int64_t)batch * b->n_slices * b->n_elems * b->n1;
-    if (n_rows_total >= (int64_t)0x7fffffff)
-        return fail(EMSIM_HIP_EUNSUPPORTED, "simulate: batch * slices * elements * n1 too large");
+    if (!mol_off_host || !elem_counts_host) return fail(EMSIM_HIP_EINVAL, "simulate: null offsets");
+    const int check = check_simulate(b, w, coords, n_atoms, batch, symmetric_bandlimit, out, mode);
+    if (check) return check;
     // host-side offsets
     if (mol_off_host[0] != 0 || mol_off_host[batch] != n_atoms)
         return fail(EMSIM_HIP_EINVAL, "simulate: molecule offsets must run from 0 to n_atoms");
@@ -1103,13 +1150,43 @@ extern "C" int emsim_hip_simulate(const emsim_hip_slice_builder* b,
     FusedWorkspace ws = fused_layout(b, batch, n_atoms, mode == 0, (char*)workspace);
     if (!workspace || workspace_bytes < ws.bytes)
         return fail(EMSIM_HIP_EWORKSPACE, "simulate: workspace needs %zu bytes", ws.bytes);
-    const int2* twq = nullptr;
-    int rc = get_twiddles_q30(&twq);
-    if (rc) return rc;
     EMSIM_HIP_CHECK(hipMemcpyAsync(ws.mol_off, mol_off_host, sizeof(int64_t) * (batch + 1),
                                    hipMemcpyHostToDevice, s));
     EMSIM_HIP_CHECK(hipMemcpyAsync(ws.elem_off, elem_off.data(), sizeof(int64_t) * elem_off.size(),
                                    hipMemcpyHostToDevice, s));
+    return simulate_core(b, w, coords, n_atoms, ws, batch, psi0, cs_mm, defocus, aperture,
+                         symmetric_bandlimit, out, slices_out, mode, s);
+}
+
+extern "C" int emsim_hip_simulate_dev(const emsim_hip_slice_builder* b,
+                                      const emsim_hip_wave_propagator* w, const float* coords,
+                                      int64_t n_atoms, const int64_t* mol_off_dev,
+                                      const int64_t* elem_off_dev, int batch, float psi0,
+                                      float cs_mm, float defocus, float aperture,
+                                      int symmetric_bandlimit, float* out, float* slices_out,
+                                      int mode, void* workspace, size_t workspace_bytes,
+                                      emsim_stream_t stream_) {
+    if (!mol_off_dev || !elem_off_dev) return fail(EMSIM_HIP_EINVAL, "simulate_dev: null offsets");
+    int rc = check_simulate(b, w, coords, n_atoms, batch, symmetric_bandlimit, out, mode);
+    if (rc) return rc;
+    FusedWorkspace ws = fused_layout(b, batch, n_atoms, mode == 0, (char*)workspace);
+    if (!workspace || workspace_bytes < ws.bytes)
+        return fail(EMSIM_HIP_EWORKSPACE, "simulate: workspace needs %zu bytes", ws.bytes);
+    ws.mol_off = const_cast<int64_t*>(mol_off_dev);
+    ws.elem_off = const_cast<int64_t*>(elem_off_dev);
+    return simulate_core(b, w, coords, n_atoms, ws, batch, psi0, cs_mm, defocus, aperture,
+                         symmetric_bandlimit, out, slices_out, mode, (hipStream_t)stream_);
+}
+
+namespace {
+
+int simulate_core(const emsim_hip_slice_builder* b, const emsim_hip_wave_propagator* w,
+                  const float* coords, int64_t n_atoms, const FusedWorkspace& ws, int batch,
+                  float psi0, float cs_mm, float defocus, float aperture, int symmetric_bandlimit,
+                  float* out, float* slices_out, int mode, hipStream_t s) {
+    const int2* twq = nullptr;
+    int rc = get_twiddles_q30(&twq);
+    if (rc) return rc;
 
     BatchGeom bg;
     bg.g.e0 = b->edges;
@@ -1177,3 +1254,5 @@ extern "C" int emsim_hip_simulate(const emsim_hip_slice_builder* b,
     if (mode == 0) return row_ifft_intensity(b->n2, wave, out, rows, b->tw, s);
     return row_fft(b->n2, true, wave, wave, rows, b->tw, s);
 }
+
+}  // namespace

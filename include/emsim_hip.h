@@ -284,6 +284,29 @@ int emsim_hip_fno_loss(const float* pred_dev, const float* target_dev, int64_t n
                        float psi0, float* loss_dev, float* dout_dev, void* workspace_dev,
                        size_t workspace_bytes, emsim_stream_t stream);
 
+/* ---- launch-shape knobs (no counterpart in the reference; set by the
+ * autotuner, emsim_amd/autotune.py) --------------------------------------
+ * "row_items_per_block": rows (row pairs) each row-worker block handles,
+ * 1..64; "col_tiles_per_block": column tiles each column-kernel block
+ * handles, 1..32; 0 restores the built-in rule.  Process-wide; they change
+ * how work is split over blocks, never the results. */
+int emsim_hip_tuning_set(const char* key, int value);
+/* The current value (0 = built-in rule), or a negative status. */
+int emsim_hip_tuning_get(const char* key);
+
+/* emsim_hip_simulate with the offsets already in device memory:
+ * mol_off_dev (batch + 1) and elem_off_dev (batch, n_elems + 1) absolute atom
+ * offsets of each molecule and of each molecule's element groups (the
+ * caller's responsibility to make consistent).  It reads no host memory and
+ * allocates nothing, so a stream capturing a HIP graph can record it (after
+ * one ordinary call has set up the library's per-device tables). */
+int emsim_hip_simulate_dev(const emsim_hip_slice_builder* b, const emsim_hip_wave_propagator* w,
+                           const float* coords_dev, int64_t n_atoms, const int64_t* mol_off_dev,
+                           const int64_t* elem_off_dev, int batch, float psi0, float cs_mm,
+                           float defocus, float aperture, int symmetric_bandlimit, float* out_dev,
+                           float* slices_out_dev, int mode, void* workspace_dev,
+                           size_t workspace_bytes, emsim_stream_t stream);
+
 /* ---- instrumentation (no counterpart in the reference, which has no
  * profiler; simulator.py:77,89-91 only logs wall time per image) ---------- */
 /* While enabled, every kernel launch is bracketed by two HIP events on its

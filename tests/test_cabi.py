@@ -53,6 +53,20 @@ def test_argument_errors_without_a_device():
     assert hip_ext.raw("emsim_hip_intensity")(None, None, 10, None) == hip_ext.EINVAL
     assert hip_ext.fft_size_supported(512) and not hip_ext.fft_size_supported(500)
     assert hip_ext.abi_version() == 2
+    # learned correction: shapes are checked before any device allocation
+    f = ctypes.c_void_p()
+    assert hip_ext.raw("emsim_hip_fno_create")(ctypes.byref(f), 512, 15, 24, 16, 4) == \
+        hip_ext.EUNSUPPORTED and not f.value
+    assert hip_ext.raw("emsim_hip_fno_create")(ctypes.byref(f), 512, 15, 32, 200, 4) == \
+        hip_ext.EINVAL
+    assert hip_ext.raw("emsim_hip_fno_create")(ctypes.byref(f), 500, 15, 32, 16, 4) == \
+        hip_ext.EUNSUPPORTED
+    d = hip_ext.GemmDesc(m=4, n=4, k=4, batch=1)        # null operands
+    assert hip_ext.raw("emsim_hip_gemm")(ctypes.addressof(d), None) == hip_ext.EINVAL
+    d = hip_ext.GemmDesc(m=4, n=4, k=4, batch=1, a=1, b=1, c=1, c_type=hip_ext.DT_BF16,
+                         split_k=2)                    # split-k partials must be float32
+    assert hip_ext.raw("emsim_hip_gemm")(ctypes.addressof(d), None) == hip_ext.EINVAL
+    assert hip_ext.raw("emsim_hip_fno_param_count")(None) == -1
 
 
 def test_backend_registry():

@@ -113,3 +113,53 @@ def test_fused_slices_output_matches_slice_builder():
     print(f"fused slices vs builder: {err:.2e}")
     assert err < 2e-6
     assert torch.equal(waves, pipe.exit_waves(batch))
+
+
+def test_graph_replay_equals_eager_and_follows_new_coords():
+    """A captured HIP graph of the fused path replays to the eager result
+    (several replays, eager work in between) and follows atoms overwritten
+    in place."""
+    tem = _tem()
+    mols = _mols([3000, 6000, 4500], 71)
+    pipe = BatchPipe(tem, 1.0, 2.0, 128, 10, ELEMS)
+    prepared = [pipe.prepare(m) for m in mols]
+    batch = pipe.upload(prepared)
+    eager = pipe.run(batch).clone()
+    g = pipe.capture(batch)
+    for _ in range(3):
+        out = g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, eager)
+        pipe.run(batch)
+        torch.cuda.synchronize()
+    quats = random_uniform_quaternions(len(mols), rng=np.random.default_rng(72))
+    prepared2 = [pipe.prepare(synth.rotated(m, q)) for m, q in zip(mols, quats)]
+    assert all((a.counts == b.counts).all() for a, b in zip(prepared, prepared2))
+    batch2 = pipe.upload(prepared2)
+    g.coords.copy_(batch2.coords)
+    out2 = g.replay().clone()
+    torch.cuda.synchronize()
+    assert torch.equal(out2, pipe.run(batch2))
+    assert not torch.equal(out2, eager)
+
+
+def test_launch_shapes_do_not_change_results():
+    from emsim_amd import autotune
+    tem = _tem()
+    mols = _mols([3000, 5000], 81)
+    pipe = BatchPipe(tem, 1.0, 2.0, 128, 8, ELEMS)
+    batch = pipe.upload([pipe.prepare(m) for m in mols])
+    autotune.apply({})
+    ref = pipe.run(batch).clone()
+    try:
+        for cfg in ({"row_items_per_block": 1, "col_tiles_per_block": 1},
+                    {"row_items_per_block": 7, "col_tiles_per_block": 5},
+                    {"row_items_per_block": 16, "col_tiles_per_block": 8}):
+            autotune.apply(cfg)
+            assert autotune.current() == cfg
+            assert torch.equal(pipe.run(batch), ref)
+        best, table = autotune.tune(lambda: pipe.run(batch), reps=1)
+        print("tuned", best, "from", len(table), "settings")
+        assert torch.equal(pipe.run(batch), ref)
+    finally:
+        autotune.apply({})
