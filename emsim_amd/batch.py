@@ -18,7 +18,7 @@ reference-shaped steps (dense histogram, ``make_multi_slices``,
 """
 from ctypes import POINTER, c_int64
 from dataclasses import dataclass
-from typing import List, Sequence
+from typing import List, Sequence, Tuple
 
 import numpy as np
 import torch
@@ -47,6 +47,56 @@ class PreparedBatch:
 
     def __len__(self):
         return len(self.labels)
+
+
+class MixedBatch:
+    """Molecules that need different wave sizes, batched by size.
+
+    Each size group runs as one fused launch sequence (``BatchPipe``); the
+    group's slice count is the reference's rule (``atoms.determine_box_size``,
+    ceil(extent / dz)) applied to the thickest molecule of the group, so
+    every molecule of a group is simulated in the same box.  Images come back
+    in input order; ``last_groups`` records (size, slice count, indices)."""
+
+    def __init__(self, microscope: TEM, pixel_size: float, slice_thickness: float,
+                 elements: Sequence[int] = (1, 6, 7, 8, 16), symmetric_bandlimit: bool = True):
+        self.microscope, self.pixel_size, self.dz = microscope, float(pixel_size), float(
+            slice_thickness)
+        self.elements = sorted(int(z) for z in elements)
+        self.symmetric_bandlimit = symmetric_bandlimit
+        self._pipes = {}
+        self.last_groups = []
+
+    def pipe(self, n: int, n_slices: int) -> "BatchPipe":
+        key = (int(n), int(n_slices))
+        if key not in self._pipes:
+            self._pipes[key] = BatchPipe(self.microscope, self.pixel_size, self.dz, n, n_slices,
+                                         self.elements, self.symmetric_bandlimit)
+        return self._pipes[key]
+
+    def slice_count(self, mols: Sequence[atm.AtomList], n: int) -> int:
+        thickest = max(float(m.space[0]) for m in mols)
+        ns, _, _ = atm.determine_box_size((thickest, 1.0, 1.0),
+                                          voxel_size=(self.dz, self.pixel_size, self.pixel_size),
+                                          box_size=(None, n, n))
+        return max(int(ns), 1)
+
+    def run(self, items: Sequence[Tuple[atm.AtomList, int]], mode: int = 0) -> List[torch.Tensor]:
+        """items: (molecule, wave size); returns one (n, n) image (mode 0) or
+        complex64 exit wave (mode 1) per item, on the GPU, in input order."""
+        out: List[torch.Tensor] = [None] * len(items)
+        self.last_groups = []
+        for n in sorted({int(sz) for _, sz in items}):
+            idx = [i for i, (_, sz) in enumerate(items) if int(sz) == n]
+            mols = [atm.centralize(items[i][0]) for i in idx]
+            ns = self.slice_count(mols, n)
+            p = self.pipe(n, ns)
+            batch = p.upload([p.prepare(m) for m in mols])
+            res = p._simulate(batch, mode)
+            for j, i in enumerate(idx):
+                out[i] = res[j]
+            self.last_groups.append((n, ns, idx))
+        return out
 
 
 class GraphRun:

@@ -83,6 +83,67 @@ def rotated(mol: AtomList, quat: np.ndarray) -> AtomList:
     return AtomList(elements=mol.elements, coordinates=out, label=mol.label)
 
 
+def water_slab(lz: float, lx: float, ly: float, density: float = 0.0334, seed: int = 1,
+               oh: float = 0.96, hoh_deg: float = 104.5, chunk: int = 1 << 21) -> AtomList:
+    """Amorphous water: oxygens uniform in the box [-lz/2, lz/2) x [-lx/2,
+    lx/2) x [-ly/2, ly/2) at `density` per A^3, two hydrogens per oxygen at
+    `oh` A and `hoh_deg` degrees in a uniformly random orientation.  Atoms
+    are ordered O, H, H per molecule; coordinates (z, x, y)."""
+    from .utils.rot import random_uniform_quaternions
+    rng = np.random.default_rng(seed)
+    n_o = int(round(density * lz * lx * ly))
+    half = 0.5 * np.deg2rad(hoh_deg)
+    # the two O-H bonds in the molecule frame
+    bonds = np.array([[np.cos(half), np.sin(half), 0.0], [np.cos(half), -np.sin(half), 0.0]]) * oh
+    out = np.empty((3 * n_o, 3), dtype=np.float32)
+    lo = np.array([-lz / 2, -lx / 2, -ly / 2])
+    hi = -lo
+    for s in range(0, n_o, chunk):
+        m = min(chunk, n_o - s)
+        o = rng.uniform(lo, hi, size=(m, 3))
+        rot = get_rotation_mattrices(random_uniform_quaternions(m, rng=rng))   # (m, 3, 3)
+        h = np.einsum("mij,bj->mbi", rot, bonds)                              # (m, 2, 3)
+        blk = out[3 * s:3 * (s + m)].reshape(m, 3, 3)
+        blk[:, 0] = o
+        blk[:, 1] = o + h[:, 0]
+        blk[:, 2] = o + h[:, 1]
+    elements = np.tile(np.array([8, 1, 1], dtype=np.int64), n_o)
+    return AtomList(elements=elements, coordinates=out, label=f"water {lz}x{lx}x{ly}")
+
+
+def large_slab(lz: float = 400.0, lx: float = 1024.0, ly: float = 1024.0, n_clouds: int = 5,
+               cloud_atoms: int = 50000, seed: int = 1) -> AtomList:
+    """Water slab plus n_clouds protein-like clouds at random positions
+    inside it (the large-slab configuration: 2048^2 waves of 0.5 A pixels,
+    400 slices of 1 A)."""
+    from .atoms import concatenate
+    slab = water_slab(lz, lx, ly, seed=seed)
+    rng = np.random.default_rng(seed + 1000)
+    parts = [slab]
+    for i in range(n_clouds):
+        c = protein_cloud(cloud_atoms, seed + 2000 + i)
+        d = 90.0 * (cloud_atoms / 50000.0) ** (1.0 / 3.0)
+        room = np.maximum(np.array([lz, lx, ly]) / 2 - d / 2, 0.0)
+        shift = rng.uniform(-room, room).astype(np.float32)
+        parts.append(AtomList(c.elements, c.coordinates + shift))
+    return concatenate(parts)
+
+
+def mixed_workload(count: int = 256, seed: int = 2, lo: int = 5000, hi: int = 200000,
+                   sizes=(128, 256, 512)):
+    """The batched mixed-size configuration: `count` clouds with atom counts
+    log-uniform in [lo, hi] (radius scaled as N^(1/3)), uniform random
+    orientations (quaternion seed `seed`) and wave sizes drawn from `sizes`.
+    Returns [(AtomList, wave size)]."""
+    from .utils.rot import random_uniform_quaternions
+    rng = np.random.default_rng(seed + 7)
+    n_atoms = np.exp(rng.uniform(np.log(lo), np.log(hi), count)).astype(int)
+    wave = rng.choice(np.asarray(sizes), size=count)
+    quats = random_uniform_quaternions(count, rng=np.random.default_rng(seed))
+    return [(rotated(protein_cloud(int(n), seed + 100 + i), quats[i]), int(w))
+            for i, (n, w) in enumerate(zip(n_atoms, wave))]
+
+
 def five_atoms() -> AtomList:
     """C, Si, Cu, Au, U on the z=0 plane at x=0, y=5..45 A (not yet centred)."""
     return AtomList(elements=np.array([6, 14, 29, 79, 92], dtype=int),

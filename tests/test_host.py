@@ -124,3 +124,13 @@ def test_pipe_and_tem_repr_and_scherzer():
     assert p.roi == (256, 256) and p.add_water is True and "ImagePipe" in repr(p)
     p.resolution = 1.0
     assert p._pixel_size == 0.5
+
+
+def test_gpu_modules_import_without_a_gpu():
+    """Every module of the package imports on a machine without a GPU (the
+    device is touched only when work is launched)."""
+    import importlib
+    for name in ("emsim_amd.batch", "emsim_amd.learned", "emsim_amd.autotune",
+                 "emsim_amd.train_fno", "emsim_amd.profiling", "emsim_amd.pipe",
+                 "emsim_amd.simulator"):
+        importlib.import_module(name)
