@@ -260,6 +260,30 @@ class BatchPipe:
             self.builder.backend.make_slices(hist, self.symmetric_bandlimit, out=V[b], ws=ws)
         return V
 
+    # -- weak-phase-object images -------------------------------------------------
+    def run_wpo(self, batch: PreparedBatch) -> torch.Tensor:
+        """Weak-phase-object images of the whole batch (``pipe.py:57-64,73-77``):
+        each molecule's 2-D projected potential (one-slice binning,
+        ``slice_builder_base.py:202-225``), then one batched transmission,
+        the dz = 0 propagation (the 2/3 band-limit alone), lens and |psi|^2."""
+        from .backend.slice_builder_hip import OneSliceBuilder
+        n, nE, B = self.n, len(self.elements), len(batch)
+        if getattr(self, "_one", None) is None:
+            self._one = OneSliceBuilder(self.elements, n, n, self.pixel_size, self.pixel_size)
+        one = self._one.backend
+        V = self._buffer("V_wpo", (B, n, n), torch.float32)
+        hist = self._buffer("hist_wpo", (nE, n, n), torch.float32)
+        ws = self._buffer("ws_wpo", (max(one.slices_workspace_bytes(), 1),), torch.uint8)
+        for b in range(B):
+            lo, hi = batch.mol_off[b], batch.mol_off[b + 1]
+            one.bin_atoms(batch.coords[lo:hi], batch.counts[b], out=hist)
+            one.make_slices(hist, self.symmetric_bandlimit, out=V[b], ws=ws)
+        mic = self.microscope
+        w0 = self.propagator.init_wave(mic.electron_dose, batch=B)
+        waves = self.propagator.singleslice_propagate(w0, V, 0.0)
+        return self.propagator.lens_intensity(waves, mic.cs_mm, mic.defocus, mic.aperture,
+                                              overwrite_wave=True)
+
     def run_unfused(self, batch: PreparedBatch) -> torch.Tensor:
         V = self.build_slices(batch)
         w0 = self.propagator.init_wave(self.microscope.electron_dose, batch=len(batch))
