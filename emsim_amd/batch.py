@@ -153,6 +153,37 @@ class BatchPipe:
         return PreparedBatch(as_device(coords, torch.float32, "coordinates"), mol_off,
                              np.ascontiguousarray(counts), [m.label for m in mols])
 
+    def upload_orientations(self, mol: atm.AtomList, quats: np.ndarray) -> PreparedBatch:
+        """One molecule in len(quats) orientations, rotated and centred on the
+        GPU: the batch that ``[prepare(rotated(mol, q)) for q in quats]``
+        would upload, with the rotation done in float32 on the device
+        (``synth.rotated`` rotates in float64 on the host, so coordinates
+        agree to float32 rounding).  The rotation is about the bounding-box
+        centre with the reference's matrix convention
+        (``utils.rot.get_rotation_mattrices``)."""
+        from .utils.rot import get_rotation_mattrices
+        quats = np.atleast_2d(np.asarray(quats, dtype=np.float64))
+        B = quats.shape[0]
+        base = self.prepare(mol)          # sorted by element; rotation keeps the order
+        n = base.coords.shape[0]
+        if n == 0:
+            raise ValueError("molecule has no atoms")
+        c = base.coords
+        centre = c.min(axis=0) + (c.max(axis=0) - c.min(axis=0)) / 2.0
+        rot = torch.as_tensor(get_rotation_mattrices(quats).astype(np.float32).reshape(B, 9))
+        dev = device()
+        out = torch.empty((B * n, 3), dtype=torch.float32, device=dev)
+        ws = torch.empty(max(int(hip_ext.raw("emsim_hip_orient_workspace_size")(B)), 1),
+                         dtype=torch.uint8, device=dev)
+        base_t = as_device(base.coords, torch.float32, "coordinates")
+        rot_t = rot.to(dev)
+        hip_ext.call("emsim_hip_orient_batch", ptr(base_t), n, ptr(rot_t), float(centre[0]),
+                     float(centre[1]), float(centre[2]), B, ptr(out), ptr(ws), ws.numel(),
+                     stream_handle())
+        mol_off = np.arange(B + 1, dtype=np.int64) * n
+        counts = np.tile(base.counts[None, :], (B, 1)).astype(np.int64)
+        return PreparedBatch(out, mol_off, np.ascontiguousarray(counts), [base.label] * B)
+
     def _buffer(self, name, shape, dtype):
         t = self._bufs.get(name)
         if t is None or tuple(t.shape) != tuple(shape) or t.dtype != dtype:

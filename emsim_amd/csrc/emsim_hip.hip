@@ -20,6 +20,7 @@
 #include "kernels_slice.hpp"
 #include "kernels_wave.hpp"
 #include "kernels_fused.hpp"
+#include "kernels_prep.hpp"
 
 using namespace emsim;
 
@@ -890,6 +891,34 @@ extern "C" int emsim_hip_intensity(const float* wave, float* out, int64_t n,
 extern "C" int emsim_hip_timing_enable(int on) {
     std::lock_guard<std::mutex> lock(g_timing_mutex);
     g_timing_on = on != 0;
+    return EMSIM_HIP_OK;
+}
+
+// ==========================================================================
+// one molecule in many orientations, on the device
+// ==========================================================================
+extern "C" size_t emsim_hip_orient_workspace_size(int batch) {
+    return batch < 1 ? 0 : align_up(sizeof(uint32_t) * 6 * (size_t)batch);
+}
+
+extern "C" int emsim_hip_orient_batch(const float* base, int64_t n_atoms, const float* rot,
+                                      float c0x, float c0y, float c0z, int batch, float* out,
+                                      void* workspace, size_t workspace_bytes,
+                                      emsim_stream_t stream_) {
+    if (!base || !rot || !out || !workspace || n_atoms < 1 || batch < 1)
+        return fail(EMSIM_HIP_EINVAL, "orient_batch: bad arguments");
+    if (batch > 65535) return fail(EMSIM_HIP_EUNSUPPORTED, "orient_batch: batch <= 65535");
+    if (workspace_bytes < emsim_hip_orient_workspace_size(batch))
+        return fail(EMSIM_HIP_EWORKSPACE, "orient_batch: workspace too small");
+    hipStream_t s = (hipStream_t)stream_;
+    uint32_t* bbox = (uint32_t*)workspace;
+    EMSIM_LAUNCH(0, k_bbox_init, dim3((unsigned)ceil_div(6 * (int64_t)batch, 256)), dim3(256), 0,
+                 s, bbox, batch);
+    EMSIM_LAUNCH(0, k_orient, dim3((unsigned)ceil_div(n_atoms, 256), (unsigned)batch), dim3(256),
+                 0, s, base, n_atoms, rot, c0x, c0y, c0z, out, bbox);
+    const int64_t total = (int64_t)batch * n_atoms * 3;
+    EMSIM_LAUNCH(0, k_recenter, dim3(grid_for(total, 256)), dim3(256), 0, s, out, n_atoms, batch,
+                 (const uint32_t*)bbox);
     return EMSIM_HIP_OK;
 }
 

@@ -284,6 +284,18 @@ int emsim_hip_fno_loss(const float* pred_dev, const float* target_dev, int64_t n
                        float psi0, float* loss_dev, float* dout_dev, void* workspace_dev,
                        size_t workspace_bytes, emsim_stream_t stream);
 
+/* ---- one molecule in many orientations ---------------------------------
+ * The device-side counterpart of emsim/atoms.py:162-211 (rotate /
+ * orientations_gen) followed by atoms.centralize (:123-159): instance b of
+ * out_dev (batch, n_atoms, 3) is rot_dev[b] (row-major 3x3 float32) applied to
+ * (x - c0) for every atom x of base_dev (n_atoms, 3), then translated so its
+ * bounding box is centred on the origin (float32, the host's operation
+ * order).  Atom order is kept, so a base sorted by element stays sorted. */
+size_t emsim_hip_orient_workspace_size(int batch);
+int emsim_hip_orient_batch(const float* base_dev, int64_t n_atoms, const float* rot_dev,
+                           float c0x, float c0y, float c0z, int batch, float* out_dev,
+                           void* workspace_dev, size_t workspace_bytes, emsim_stream_t stream);
+
 /* ---- launch-shape knobs (no counterpart in the reference; set by the
  * autotuner, emsim_amd/autotune.py) --------------------------------------
  * "row_items_per_block": rows (row pairs) each row-worker block handles,
