@@ -88,6 +88,8 @@ _SIGNATURES = {
     "emsim_hip_fno_adam": (c_int, [c_void_p, c_float, c_float, c_float, c_float, c_float, c_int,
                                    c_void_p]),
     "emsim_hip_fno_force_generic": (c_int, [c_int]),
+    "emsim_hip_fno_infer": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_float, c_float,
+                                    c_void_p, c_int, c_void_p, c_size_t, c_void_p]),
     "emsim_hip_fno_apply": (c_int, [c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_int,
                                     c_void_p]),
     "emsim_hip_fno_loss_workspace_size": (c_size_t, [c_int64, c_int]),

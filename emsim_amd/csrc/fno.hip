@@ -385,6 +385,7 @@ int spectral_middle(const emsim_hip_fno* f, const FnoWs& w, int batch, int l, fl
 
 // the row-kernel modes the forward pass uses
 constexpr int kModeLift = kRowV | kRowT;
+constexpr int kModeLiftPacked = kModeLift | kRowPack;
 constexpr int kModeMid = kRowU | kRowAct | kRowV | kRowT;
 constexpr int kModeMidTrain = kModeMid | kRowZ;
 constexpr int kModeLast = kRowU | kRowProj;
@@ -407,6 +408,17 @@ int launch_rows_mode(const FnoRowArgs& a, int mode, dim3 grid, size_t lds, hipSt
 int launch_rows(const FnoRowArgs& a, int mode, int64_t rows, hipStream_t s) {
     const dim3 grid((unsigned)rows);
     const size_t lds = kFnoRowLds(a.n);
+    if (mode == kModeLiftPacked) {
+        if (a.k1 == 16 && a.m2x2 == 16)
+            EMSIM_LAUNCH(0, (k_fno_rows<1, 1, kModeLiftPacked>), grid, dim3(256), lds, s, a);
+        else if (a.k1 == 16)
+            EMSIM_LAUNCH(0, (k_fno_rows<1, 2, kModeLiftPacked>), grid, dim3(256), lds, s, a);
+        else if (a.m2x2 == 16)
+            EMSIM_LAUNCH(0, (k_fno_rows<2, 1, kModeLiftPacked>), grid, dim3(256), lds, s, a);
+        else
+            EMSIM_LAUNCH(0, (k_fno_rows<2, 2, kModeLiftPacked>), grid, dim3(256), lds, s, a);
+        return EMSIM_HIP_OK;
+    }
     if (a.k1 == 16) {
         if (mode != kModeLift) return fail(EMSIM_HIP_EINVAL, "fno rows: 16 input channels only lift");
         if (a.m2x2 == 16)
@@ -419,8 +431,17 @@ int launch_rows(const FnoRowArgs& a, int mode, int64_t rows, hipStream_t s) {
                         : launch_rows_mode<2, 2>(a, mode, grid, lds, s);
 }
 
+// The lifting reads either the packed input x, or (x null) the exit waves
+// and potentials directly.
+struct PackedInput {
+    const float2* wave;
+    const float* slices;
+    int ns;
+    float inv_psi0, sigma;
+};
+
 int forward_fused(const emsim_hip_fno* f, const FnoWs& w, const void* x, float* out, int batch,
-                  int training, hipStream_t s) {
+                  int training, hipStream_t s, const PackedInput* packed = nullptr) {
     const int64_t rows = (int64_t)batch * f->n;
     FnoRowArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -435,7 +456,14 @@ int forward_fused(const emsim_hip_fno* f, const FnoWs& w, const void* x, float* 
     a.bias = f->params + f->off_lb;
     a.vout = w.v[0];
     a.fy = f->fy;
-    int rc = launch_rows(a, kModeLift, rows, s);
+    if (packed) {
+        a.wave = packed->wave;
+        a.slices = packed->slices;
+        a.ns = packed->ns;
+        a.inv_psi0 = packed->inv_psi0;
+        a.sigma = packed->sigma;
+    }
+    int rc = launch_rows(a, packed ? kModeLiftPacked : kModeLift, rows, s);
     if (rc) return rc;
     for (int l = 0; l < f->L; ++l) {
         const bool last = l + 1 == f->L;
@@ -617,6 +645,23 @@ int emsim_hip_fno_backward(emsim_hip_fno* f, const void* x_dev, const float* dou
                           C, w.dout_partial, g + f->off_lw, s)))
         return rc;
     return colsum(w.dv, P, C, w.colsum_part, g + f->off_lb, s);
+}
+
+int emsim_hip_fno_infer(emsim_hip_fno* f, const float* wave_dev, const float* slices_dev,
+                        int n_slices, float psi0, float sigma, float* out_dev, int batch,
+                        void* workspace_dev, size_t workspace_bytes, emsim_stream_t stream) {
+    if (!f || !wave_dev || !out_dev || !workspace_dev || (n_slices > 0 && !slices_dev))
+        return fail(EMSIM_HIP_EINVAL, "fno_infer: null argument");
+    if (batch < 1 || !(psi0 > 0.f) || n_slices < 0 || 2 + n_slices != f->cin)
+        return fail(EMSIM_HIP_EINVAL, "fno_infer: bad batch, psi0 or slice count");
+    if (!fused_rows_ok(f) || g_fno_force_generic)
+        return fail(EMSIM_HIP_EUNSUPPORTED,
+                    "fno_infer: this network shape needs fno_pack_input + fno_forward");
+    const FnoWs w = carve(f, batch, 0, (char*)workspace_dev);
+    if (workspace_bytes < w.bytes)
+        return fail(EMSIM_HIP_EWORKSPACE, "fno_infer: workspace %zu < %zu", workspace_bytes, w.bytes);
+    const PackedInput p{(const float2*)wave_dev, slices_dev, n_slices, 1.f / psi0, sigma};
+    return forward_fused(f, w, nullptr, out_dev, batch, 0, (hipStream_t)stream, &p);
 }
 
 int emsim_hip_fno_force_generic(int on) {

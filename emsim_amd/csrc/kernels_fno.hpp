@@ -196,6 +196,12 @@ struct FnoRowArgs {
     const float* qb;       // (2)
     float* out;            // (rows, n, 2) float32
     int n;
+    // lifting straight from the simulation's outputs (mode bit kRowPack):
+    // channels [Re psi / psi0, Im psi / psi0, sigma V_1 .. sigma V_ns, 0 ...]
+    const float2* wave;    // (B, n, n)
+    const float* slices;   // (B, ns, n, n)
+    int ns;
+    float inv_psi0, sigma;
 };
 
 constexpr int kFnoWStride = 72;    // bf16 per LDS row of W^T / Uhat^T (144 B, 16-B aligned)
@@ -215,7 +221,33 @@ __host__ __device__ constexpr size_t kFnoRowLds(int n) {
 // overlaps the staging of W^T and Uhat^T.
 // MODE bits: the sum has the Uhat term, GELU, v' written, z written, the
 // next y-transform computed, the projection computed.
-constexpr int kRowU = 1, kRowAct = 2, kRowV = 4, kRowZ = 8, kRowT = 16, kRowProj = 32;
+constexpr int kRowU = 1, kRowAct = 2, kRowV = 4, kRowZ = 8, kRowT = 16, kRowProj = 32,
+              kRowPack = 64;
+
+// Lifting input channels 16 f + 8 h .. + 7 of pixel (row, y), built from the
+// exit wave and potentials exactly as k_fno_pack writes them.
+__device__ __forceinline__ bf16x8_t fno_packed_frag(const FnoRowArgs& a, int64_t row, int y, int f,
+                                                    int h) {
+    const int64_t b = row / a.n, x = row - b * a.n;
+    const int64_t npix = (int64_t)a.n * a.n;
+    float v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int c = 16 * f + 8 * h + e;
+        float val = 0.f;
+        if (c < 2) {
+            const float2 w = a.wave[row * a.n + y];
+            val = (c == 0 ? w.x : w.y) * a.inv_psi0;
+        } else if (c < 2 + a.ns) {
+            val = a.sigma * a.slices[(b * a.ns + (c - 2)) * npix + x * a.n + y];
+        }
+        v[e] = val;
+    }
+    uint32_t q[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) q[j] = f32x2_to_bf16x2(v[2 * j], v[2 * j + 1]);
+    return __builtin_bit_cast(bf16x8_t, q);
+}
 
 template <int K1F, int K2F, int MODE>
 __global__ __launch_bounds__(256) void k_fno_rows(FnoRowArgs a) {
@@ -240,7 +272,10 @@ __global__ __launch_bounds__(256) void k_fno_rows(FnoRowArgs a) {
 #pragma unroll
         for (int f = 0; f < K1F; ++f) {
             vf[i][f] = bf16x8_t{};
-            if (ok) vf[i][f] = *(const bf16x8_t*)(vrow + (int64_t)y * k1 + 16 * f + 8 * h);
+            if (ok) {
+                if constexpr (MODE & kRowPack) vf[i][f] = fno_packed_frag(a, row, y, f, h);
+                else vf[i][f] = *(const bf16x8_t*)(vrow + (int64_t)y * k1 + 16 * f + 8 * h);
+            }
         }
 #pragma unroll
         for (int f = 0; f < K2F; ++f) {

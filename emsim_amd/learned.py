@@ -197,6 +197,26 @@ class FNO:
             raise ValueError(f"{name} must be a contiguous {dtype} GPU tensor of shape "
                              f"{tuple(shape)}, got {t.dtype} {tuple(t.shape)}")
 
+    def infer(self, wave: torch.Tensor, slices: torch.Tensor, psi0: float, sigma: float,
+              out: torch.Tensor = None) -> torch.Tensor:
+        """forward(pack(wave, slices, ...)) with the packing fused into the
+        lifting (same values); falls back to the two steps for network shapes
+        the fused kernels do not serve."""
+        B, n = wave.shape[0], self.n
+        self._check(wave, (B, n, n), torch.complex64, "wave")
+        self._check(slices, (B, self.cin - 2, n, n), torch.float32, "slices")
+        if out is None:
+            out = torch.empty((B, n, n, 2), dtype=torch.float32, device=device())
+        self._check(out, (B, n, n, 2), torch.float32, "out")
+        ws = self.workspace(B, False)
+        rc = hip_ext.raw("emsim_hip_fno_infer")(self.h, ptr(wave), ptr(slices), self.cin - 2,
+                                               float(psi0), float(sigma), ptr(out), B, ptr(ws),
+                                               ws.numel(), stream_handle())
+        if rc == hip_ext.EUNSUPPORTED:
+            return self.forward(self.pack(wave, slices, psi0, sigma), out=out)
+        hip_ext.check(rc, "emsim_hip_fno_infer")
+        return out
+
     def backward(self, x: torch.Tensor, dout: torch.Tensor) -> None:
         B = x.shape[0]
         self._check(x, (B, self.n, self.n, self.cin_pad), torch.bfloat16, "x")
@@ -357,8 +377,7 @@ class LearnedPipe:
             one.coords.record_stream(self._side)
             sampled = (s, fine_s)
         cw, cs, _ = p.make(batch, fine=False)
-        x = self.fno.pack(cw, cs, p.psi0, p.sigma)
-        out = self.fno.forward(x)
+        out = self.fno.infer(cw, cs, p.psi0, p.sigma)
         wave = self.fno.apply(cw, out, p.psi0)
         rep = GateReport()
         if not check:

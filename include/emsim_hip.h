@@ -261,6 +261,14 @@ int emsim_hip_fno_pack_input(const emsim_hip_fno* f, const float* wave_dev,
 int emsim_hip_fno_forward(emsim_hip_fno* f, const void* x_dev, float* out_dev, int batch,
                           int training, void* workspace_dev, size_t workspace_bytes,
                           emsim_stream_t stream);
+/* Inference straight from the simulation: emsim_hip_fno_pack_input fused
+ * into the lifting (the same values, no packed tensor in memory).  Same
+ * workspace as an inference forward pass.  Returns EMSIM_HIP_EUNSUPPORTED
+ * for network shapes the fused row kernels do not serve (use pack_input +
+ * forward then). */
+int emsim_hip_fno_infer(emsim_hip_fno* f, const float* wave_dev, const float* slices_dev,
+                        int n_slices, float psi0, float sigma, float* out_dev, int batch,
+                        void* workspace_dev, size_t workspace_bytes, emsim_stream_t stream);
 /* on != 0: every forward pass uses the generic GEMM sequence instead of the
  * fused row kernels (which serve 32 channels, <= 16 modes, n <= 512); the
  * two compute the same network, this selects one for cross-checks. */

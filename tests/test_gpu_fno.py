@@ -206,6 +206,19 @@ def test_fused_rows_equal_generic_path(hip, net):
     assert rel < 1e-3
 
 
+def test_infer_equals_pack_then_forward(net):
+    """Packing fused into the lifting gives the same output bit for bit."""
+    f, _, _, _ = net
+    rng = np.random.default_rng(12)
+    ns = CIN - 2
+    wave = _dev((rng.normal(size=(BB, NN, NN)) + 1j * rng.normal(size=(BB, NN, NN)))
+                .astype(np.complex64), torch.complex64)
+    sl = _dev(rng.normal(size=(BB, ns, NN, NN)).astype(np.float32))
+    a = f.infer(wave, sl, 0.8, 0.035)
+    b = f.forward(f.pack(wave, sl, 0.8, 0.035))
+    assert torch.equal(a, b)
+
+
 def test_backward_matches_oracle(net):
     f, p, xt, x_bf = net
     rng = np.random.default_rng(9)
