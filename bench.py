@@ -97,6 +97,16 @@ def learned_bench(args, cfg, tem, mols, plain_ips):
     kernels = hip_ext.timing_collect()
     assert torch.isfinite(img).all().item()
     value = args.batch * args.steps / elapsed
+    # the same with the sampled fine-slice check on every 4th batch only
+    # (norm check still on every image)
+    lp4 = LearnedPipe(fno, pairs, check_every=4)
+    lp4.run(batch)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(max(args.steps, 4)):
+        img4, rep4 = lp4.run(batch)
+    torch.cuda.synchronize()
+    value4 = args.batch * max(args.steps, 4) / (time.perf_counter() - t0)
     # accuracy on the benchmark batch (outside the timed region)
     wave, _ = lp.exit_waves(batch, check=False)
     cw, _, fine = pairs.make(batch)
@@ -115,6 +125,8 @@ def learned_bench(args, cfg, tem, mols, plain_ips):
     return {
         "images_per_s": round(value, 2),
         "speedup_vs_plain": round(value / plain_ips, 3),
+        "images_per_s_fine_check_every_4th_batch": round(value4, 2),
+        "speedup_vs_plain_fine_check_every_4th_batch": round(value4 / plain_ips, 3),
         "ms_per_step": round(elapsed / args.steps * 1e3, 3),
         "slices_per_s_fine_equivalent": round(value * cfg["slices"], 1),
         "coarse_dz": coarse_dz, "coarse_slices": coarse_slices,
