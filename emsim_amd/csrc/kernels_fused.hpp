@@ -94,10 +94,9 @@ __global__ void k_csr_scatter(int64_t n_atoms, const int32_t* __restrict__ atom_
 // One block per (column tile, molecule): a 1-D grid of n_tiles * batch.
 // Y rows are ldh complex apart, images n1 * ldh apart.
 // ---------------------------------------------------------------------------
-// The (molecule, slice)'s row table (n_elems * N + 1 entries, when it fits),
-// up to kSpecBucketLds atom y-bins and, for n2 <= kSpecTwLds, the Q30
-// twiddles are staged in LDS once per block; anything larger is read from
-// global memory.
+// When the (molecule, slice)'s row table (n_elems * N + 1 entries), its atom
+// y-bins and the Q30 twiddles (n2 entries) all fit, they are staged in LDS
+// once per block; otherwise the row DFTs read all three from global memory.
 constexpr int kSpecRowsLds = 4097;
 constexpr int kSpecBucketLds = 8192;
 constexpr int kSpecTwLds = 1024;
@@ -117,6 +116,32 @@ __device__ __forceinline__ unsigned spec_item(unsigned L, unsigned total) {
 #ifndef EMSIM_SPEC_MIN_WAVES
 #define EMSIM_SPEC_MIN_WAVES 4
 #endif
+
+// Row DFTs of this thread's EPT rows r0 + m T at column col: the Q30
+// twiddles w^(col y) of the row's atoms summed in 64-bit integers.  rows
+// holds absolute bucket positions (minus base: the position of bkt[0]);
+// twiddle p sits at tws[p * tw_step].
+template <int EPT, int T>
+__device__ __forceinline__ void row_dft(float2 (&v)[EPT], bool valid, int r0, int col, int mask,
+                                        const uint32_t* __restrict__ rows, uint32_t base,
+                                        const uint16_t* __restrict__ bkt,
+                                        const int2* __restrict__ tws, int tw_step) {
+#pragma unroll
+    for (int m = 0; m < EPT; ++m) {
+        int64_t sr = 0, si = 0;
+        if (valid) {
+            const int r = r0 + m * T;
+            const uint32_t beg = rows[r] - base, end = rows[r + 1] - base;
+            for (uint32_t k = beg; k < end; ++k) {
+                const int p = (col * (int)bkt[k]) & mask;
+                const int2 w = tws[p * tw_step];
+                sr += w.x;
+                si += w.y;
+            }
+        }
+        v[m] = make_float2((float)sr * kQ30Inv, (float)si * kQ30Inv);
+    }
+}
 
 template <int N>
 __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_SPEC_MIN_WAVES : 1) void k_spec_sparse(
@@ -156,8 +181,10 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
     auto row_at = [&](int i) -> uint32_t { return rows_lds ? lds_rows[i] : row_start[rbase + i]; };
     const uint32_t kbase = row_at(0);
     const uint32_t kcount = row_at(n_rows) - kbase;
-    const uint32_t kstaged = kcount < (uint32_t)kSpecBucketLds ? kcount : (uint32_t)kSpecBucketLds;
-    for (uint32_t i = threadIdx.x; i < kstaged; i += Cfg::BLOCK) lds_bucket[i] = bucket[kbase + i];
+    // everything in LDS (the usual case), or everything from global memory
+    const bool all_lds = rows_lds && tw_lds && kcount <= (uint32_t)kSpecBucketLds;
+    if (all_lds)
+        for (uint32_t i = threadIdx.x; i < kcount; i += Cfg::BLOCK) lds_bucket[i] = bucket[kbase + i];
     __syncthreads();
 
     float2 acc[EPT];
@@ -166,22 +193,13 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
 
     for (int e = 0; e < n_elems; ++e) {
         float2 v[EPT];
-#pragma unroll
-        for (int m = 0; m < EPT; ++m) {
-            int64_t sr = 0, si = 0;
-            if (valid) {
-                const int r = e * N + t + m * Cfg::T;
-                const uint32_t beg = row_at(r) - kbase, end = row_at(r + 1) - kbase;
-                for (uint32_t k = beg; k < end; ++k) {
-                    const int y = k < kstaged ? (int)lds_bucket[k] : (int)bucket[kbase + k];
-                    const int p = (col * y) & (n2 - 1);
-                    const int2 w = tw_lds ? lds_twq[p] : twq[p * tw_step];
-                    sr += w.x;
-                    si += w.y;
-                }
-            }
-            v[m] = make_float2((float)sr * kQ30Inv, (float)si * kQ30Inv);
-        }
+        // block-uniform choice, so each path's loads have one address space
+        if (all_lds)
+            row_dft<EPT, Cfg::T>(v, valid, e * N + t, col, n2 - 1, lds_rows, kbase, lds_bucket,
+                                 lds_twq, 1);
+        else
+            row_dft<EPT, Cfg::T>(v, valid, e * N + t, col, n2 - 1, row_start + rbase, 0u, bucket,
+                                 twq, tw_step);
         fft_n<N, EPT, false>(v, re, im, t, tw);
         const float* f = sf + (int64_t)e * sf_plane + col;
 #pragma unroll
