@@ -317,7 +317,8 @@ int launch_col_mul_impl(float2* data, int ld, int ncols, int64_t img_stride, int
     const int tiles_per_img = (int)ceil_div(ncols, Cfg::C);
     const int64_t n_tiles = (int64_t)tiles_per_img * n_img;
     const int64_t tpb = col_tiles_per_block(n_tiles);
-    const dim3 grid((unsigned)ceil_div(n_tiles, tpb));
+    // blocks pair up on an XCD (see k_col_fft_mul_ifft): a multiple of 16
+    const dim3 grid((unsigned)(ceil_div(ceil_div(n_tiles, tpb), 16) * 16));
     EMSIM_LAUNCH(N, (k_col_fft_mul_ifft<N, Mul>), grid, dim3(Cfg::BLOCK), 0, s, data, ld, ncols,
                  tiles_per_img, img_stride, n_tiles, (int)tpb, mul, tw);
     return EMSIM_HIP_OK;

@@ -178,9 +178,15 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
     if (tw_lds)
         for (int i = threadIdx.x; i < n2; i += Cfg::BLOCK) lds_twq[i] = twq[i * tw_step];
     __syncthreads();
-    auto row_at = [&](int i) -> uint32_t { return rows_lds ? lds_rows[i] : row_start[rbase + i]; };
-    const uint32_t kbase = row_at(0);
-    const uint32_t kcount = row_at(n_rows) - kbase;
+    uint32_t kbase, kend;
+    if (rows_lds) {
+        kbase = lds_rows[0];
+        kend = lds_rows[n_rows];
+    } else {
+        kbase = row_start[rbase];
+        kend = row_start[rbase + n_rows];
+    }
+    const uint32_t kcount = kend - kbase;
     // everything in LDS (the usual case), or everything from global memory
     const bool all_lds = rows_lds && tw_lds && kcount <= (uint32_t)kSpecBucketLds;
     if (all_lds)

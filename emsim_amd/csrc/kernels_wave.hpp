@@ -309,10 +309,24 @@ inline int rows_per_block(int64_t n_groups) {
 // ---------------------------------------------------------------------------
 // column kernel: FFT_col -> pointwise Mul(k1, j) -> IFFT_col, in place.
 // n_img images of N rows x ld complex (image stride img_stride); the first
-// ncols columns of each image are transformed in tiles of C columns.  Block
-// b handles tiles b*tpb .. b*tpb+tpb-1 (tiles numbered image-major) and
-// loads the next tile while transforming the current one.
+// ncols columns of each image are transformed in tiles of C columns (tiles
+// numbered image-major).  Each block handles tpb tiles and loads the next
+// tile while transforming the current one.
+//
+// Workgroups go to the 8 XCDs round-robin by linear id, so blocks L and L + 8
+// share an XCD and start together.  They take the even and the odd tiles of
+// the same strips of 2 C columns (128 bytes of a row when C = 8): block pair
+// p = (L / 16) * 8 + L % 8 owns tiles 2 tpb p .. 2 tpb p + 2 tpb - 1, and
+// (L / 8) % 2 picks the parity.  Both halves of a cache line are then read
+// while the line is in that XCD's L2, instead of the second half being
+// fetched again from HBM a tile later.  The grid is a multiple of 16.
 // ---------------------------------------------------------------------------
+constexpr unsigned kColXcds = 8;
+__device__ __forceinline__ int64_t col_tile(unsigned L, int tpb, int r) {
+    const unsigned x = L % kColXcds, k = L / kColXcds;
+    const int64_t pair = (int64_t)(k / 2) * kColXcds + x;
+    return (pair * tpb + r) * 2 + (k % 2);
+}
 template <int N, class Mul>
 __global__ __launch_bounds__(ColCfg<N>::BLOCK, EMSIM_COL_MIN_WAVES) void k_col_fft_mul_ifft(
     float2* data, int ld, int ncols, int tiles_per_img, int64_t img_stride, int64_t n_tiles,
@@ -335,13 +349,12 @@ __global__ __launch_bounds__(ColCfg<N>::BLOCK, EMSIM_COL_MIN_WAVES) void k_col_f
         ok = g < n_tiles && col < ncols;
         return data + img * img_stride + col;
     };
-    const int64_t g0 = (int64_t)blockIdx.x * tpb;
     TwCache<N, EPT> twc;
     twc.load(tw, t);
     float2 cur[EPT], nxt[EPT];
     int col;
     bool ok;
-    float2* base = locate(g0, col, ok);
+    float2* base = locate(col_tile(blockIdx.x, tpb, 0), col, ok);
 #pragma unroll
     for (int m = 0; m < EPT; ++m)
         cur[m] = ok ? base[(int64_t)(t + m * Cfg::T) * ld] : make_float2(0.f, 0.f);
@@ -351,7 +364,7 @@ __global__ __launch_bounds__(ColCfg<N>::BLOCK, EMSIM_COL_MIN_WAVES) void k_col_f
         bool ok_n = false;
         float2* base_n = nullptr;
         if (r + 1 < tpb) {
-            base_n = locate(g0 + r + 1, col_n, ok_n);
+            base_n = locate(col_tile(blockIdx.x, tpb, r + 1), col_n, ok_n);
 #pragma unroll
             for (int m = 0; m < EPT; ++m)
                 nxt[m] = ok_n ? base_n[(int64_t)(t + m * Cfg::T) * ld] : make_float2(0.f, 0.f);
