@@ -156,6 +156,22 @@ __device__ __forceinline__ float2 tw_get(const TwCache<N, EPT>& c, int q, int r,
     return c.w[TwCache<N, EPT>::pass_of(Ns)][q][r - 1];
 }
 
+// The N twiddles w_N^j of one transform length staged in LDS (p[j], j < N):
+// every twiddle of every pass at a low-latency load and no registers.
+template <int LEN>
+struct TwLds {
+    const float2* p;
+    // cooperative fill from the table; the caller syncs before use
+    __device__ __forceinline__ static void fill(float2* dst, const float2* __restrict__ tw,
+                                                int tid, int nthreads) {
+        for (int j = tid; j < LEN; j += nthreads) dst[j] = tw[j * (kTwN / LEN)];
+    }
+};
+template <int N, int EPT, int Ns, int R>
+__device__ __forceinline__ float2 tw_get(const TwLds<N>& c, int q, int r, int k) {
+    return c.p[k * r * (N / (Ns * R))];
+}
+
 // One Stockham pass: radix R over sub-transforms of length Ns, EPT values
 // per thread.  Before: v[m] = x[t + m N/EPT].  Butterfly j = t + q N/EPT
 // (q < EPT/R) reads x[j + r N/R] (= v[q + r EPT/R]) and writes

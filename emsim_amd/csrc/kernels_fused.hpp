@@ -100,6 +100,12 @@ __global__ void k_csr_scatter(int64_t n_atoms, const int32_t* __restrict__ atom_
 constexpr int kSpecRowsLds = 4097;
 constexpr int kSpecBucketLds = 8192;
 constexpr int kSpecTwLds = 1024;
+// Element planes with at most this many atoms in the slice take the direct
+// 2-D DFT instead of row DFTs + a column transform.
+#ifndef EMSIM_SPEC_DIRECT
+#define EMSIM_SPEC_DIRECT 4
+#endif
+constexpr int kSpecDirect = EMSIM_SPEC_DIRECT;
 // The dispatcher hands workgroups to the 8 XCDs round-robin by linear id.
 // Block L therefore runs work item spec_item(L): XCD x gets one contiguous,
 // tile-major range of items, so the blocks of a column tile share that XCD's
@@ -143,6 +149,14 @@ __device__ __forceinline__ void row_dft(float2 (&v)[EPT], bool valid, int r0, in
     }
 }
 
+// Column transform of the spectrum kernel, twiddles from LDS or the table.
+template <int N, int EPT, bool INV, bool LDS>
+__device__ __forceinline__ void spec_fft(float2 (&v)[EPT], float* re, float* im, int t,
+                                         const float2* lds_tw, const float2* __restrict__ tw) {
+    if constexpr (LDS) fft_n<N, EPT, INV>(v, re, im, t, TwLds<N>{lds_tw});
+    else fft_n<N, EPT, INV>(v, re, im, t, tw);
+}
+
 template <int N>
 __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_SPEC_MIN_WAVES : 1) void k_spec_sparse(
     const uint32_t* __restrict__ row_start, const uint16_t* __restrict__ bucket,
@@ -156,6 +170,12 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
     __shared__ uint32_t lds_rows[kSpecRowsLds];
     __shared__ uint16_t lds_bucket[kSpecBucketLds];
     __shared__ int2 lds_twq[kSpecTwLds];
+    __shared__ uint32_t lds_atoms_raw[kSpecDirect > 0 ? kSpecDirect : 1];   // (x << 16) | y
+    __shared__ uint32_t lds_atoms[kSpecDirect > 0 ? kSpecDirect : 1];       // the same, sorted
+    // the column transforms' twiddles (N <= 512; longer columns read the table,
+    // as the LDS would then no longer hold two blocks per CU)
+    constexpr bool kTwInLds = N <= 512;
+    __shared__ float2 lds_tw[kTwInLds ? N : 1];
     const unsigned item = spec_item(blockIdx.x, gridDim.x);
     const int tile = (int)(item / (unsigned)batch);
     const int b = (int)(item % (unsigned)batch);
@@ -177,14 +197,17 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
     const bool tw_lds = n2 <= kSpecTwLds;
     if (tw_lds)
         for (int i = threadIdx.x; i < n2; i += Cfg::BLOCK) lds_twq[i] = twq[i * tw_step];
+    if constexpr (kTwInLds) TwLds<N>::fill(lds_tw, tw, threadIdx.x, Cfg::BLOCK);
     __syncthreads();
+    // block-uniform values read per lane: readfirstlane puts them in scalar
+    // registers, so the branches on them are scalar branches
     uint32_t kbase, kend;
     if (rows_lds) {
-        kbase = lds_rows[0];
-        kend = lds_rows[n_rows];
+        kbase = __builtin_amdgcn_readfirstlane(lds_rows[0]);
+        kend = __builtin_amdgcn_readfirstlane(lds_rows[n_rows]);
     } else {
-        kbase = row_start[rbase];
-        kend = row_start[rbase + n_rows];
+        kbase = __builtin_amdgcn_readfirstlane(row_start[rbase]);
+        kend = __builtin_amdgcn_readfirstlane(row_start[rbase + n_rows]);
     }
     const uint32_t kcount = kend - kbase;
     // everything in LDS (the usual case), or everything from global memory
@@ -197,23 +220,95 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
 #pragma unroll
     for (int m = 0; m < EPT; ++m) acc[m] = make_float2(0.f, 0.f);
 
+    bool any = false;   // block-uniform: some element has atoms in this slice
     for (int e = 0; e < n_elems; ++e) {
-        float2 v[EPT];
-        // block-uniform choice, so each path's loads have one address space
-        if (all_lds)
-            row_dft<EPT, Cfg::T>(v, valid, e * N + t, col, n2 - 1, lds_rows, kbase, lds_bucket,
-                                 lds_twq, 1);
-        else
-            row_dft<EPT, Cfg::T>(v, valid, e * N + t, col, n2 - 1, row_start + rbase, 0u, bucket,
-                                 twq, tw_step);
-        fft_n<N, EPT, false>(v, re, im, t, tw);
+        uint32_t e_beg, e_end;
+        if (rows_lds) {
+            e_beg = __builtin_amdgcn_readfirstlane(lds_rows[e * N]);
+            e_end = __builtin_amdgcn_readfirstlane(lds_rows[(e + 1) * N]);
+        } else {
+            e_beg = __builtin_amdgcn_readfirstlane(row_start[rbase + e * N]);
+            e_end = __builtin_amdgcn_readfirstlane(row_start[rbase + (e + 1) * N]);
+        }
+        const uint32_t n_e = e_end - e_beg;
+        // an element without atoms in this slice adds exactly zero
+        if (n_e == 0) continue;
+        any = true;
+        // form factors first: their loads stay in flight behind the transform
         const float* f = sf + (int64_t)e * sf_plane + col;
+        float fe[EPT];
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) fe[m] = valid ? f[(int64_t)(t + m * Cfg::T) * H] : 0.f;
+        float2 v[EPT];
+        if (all_lds && n_e <= (uint32_t)kSpecDirect) {
+            // few atoms: their 2-D DFT sum_a w_N^(k x_a) w_n2^(col y_a) directly,
+            // cheaper than a column transform (the y factor is the Q30 twiddle
+            // the row DFT would use, the x factor the transform's table entry)
+            if (c == 0) {
+#pragma unroll
+                for (int m = 0; m < EPT; ++m) {
+                    const int x = t + m * Cfg::T;
+                    const uint32_t beg = lds_rows[e * N + x], end = lds_rows[e * N + x + 1];
+                    for (uint32_t k = beg; k < end; ++k)
+                        lds_atoms_raw[k - e_beg] = ((uint32_t)x << 16) | lds_bucket[k - kbase];
+                }
+            }
+            __syncthreads();
+            // sorted by (x, y), so the float sum below does not depend on the
+            // order the atoms landed in their buckets (atoms with equal keys
+            // contribute identical terms)
+            if (threadIdx.x < n_e) {
+                const uint32_t key = lds_atoms_raw[threadIdx.x];
+                uint32_t rank = 0;
+                for (uint32_t j = 0; j < n_e; ++j) {
+                    const uint32_t o = lds_atoms_raw[j];
+                    rank += (o < key || (o == key && j < threadIdx.x)) ? 1u : 0u;
+                }
+                lds_atoms[rank] = key;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < EPT; ++m) v[m] = make_float2(0.f, 0.f);
+            for (uint32_t a = 0; a < n_e; ++a) {
+                const uint32_t xy = lds_atoms[a];
+                const int x = (int)(xy >> 16), y = (int)(xy & 0xffffu);
+                const int2 q = lds_twq[(col * y) & (n2 - 1)];
+                const float2 wy = make_float2((float)q.x * kQ30Inv, (float)q.y * kQ30Inv);
+#pragma unroll
+                for (int m = 0; m < EPT; ++m) {
+                    const int p = ((t + m * Cfg::T) * x) & (N - 1);
+                    float2 wx;
+                    if constexpr (kTwInLds) wx = lds_tw[p];
+                    else wx = tw[p * (kTwN / N)];
+                    v[m].x += wx.x * wy.x - wx.y * wy.y;
+                    v[m].y += wx.x * wy.y + wx.y * wy.x;
+                }
+            }
+            __syncthreads();   // the next element rebuilds lds_atoms
+        } else {
+            // block-uniform choice, so each path's loads have one address space
+            if (all_lds)
+                row_dft<EPT, Cfg::T>(v, valid, e * N + t, col, n2 - 1, lds_rows, kbase, lds_bucket,
+                                     lds_twq, 1);
+            else
+                row_dft<EPT, Cfg::T>(v, valid, e * N + t, col, n2 - 1, row_start + rbase, 0u,
+                                     bucket, twq, tw_step);
+            spec_fft<N, EPT, false, kTwInLds>(v, re, im, t, lds_tw, tw);
+        }
 #pragma unroll
         for (int m = 0; m < EPT; ++m) {
-            const float fe = valid ? f[(int64_t)(t + m * Cfg::T) * H] : 0.f;
-            acc[m].x += v[m].x * fe;
-            acc[m].y += v[m].y * fe;
+            acc[m].x += v[m].x * fe[m];
+            acc[m].y += v[m].y * fe[m];
         }
+    }
+    if (!any) {
+        // empty slice of this molecule: the half spectrum is exactly zero
+        if (valid) {
+            float2* dst = Y + (int64_t)b * N * ldh + col;
+#pragma unroll
+            for (int m = 0; m < EPT; ++m) dst[(int64_t)(t + m * Cfg::T) * ldh] = make_float2(0.f, 0.f);
+        }
+        return;
     }
     const int kj = freq_int(col, n2);
 #pragma unroll
@@ -226,7 +321,7 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
         }
         acc[m] = make_float2(acc[m].x * s, acc[m].y * s);
     }
-    fft_n<N, EPT, true>(acc, re, im, t, tw);
+    spec_fft<N, EPT, true, kTwInLds>(acc, re, im, t, lds_tw, tw);
     if (valid) {
         float2* dst = Y + (int64_t)b * N * ldh + col;
 #pragma unroll

@@ -71,6 +71,53 @@ def test_fused_batch_matches_oracle(n, ns, dz, pixel):
         assert ei < TOL and ei64 < TOL and ex < TOL and ex64 < TOL
 
 
+def _sparse_planes_molecule(seed=7):
+    """Element planes of every kind the spectrum pass distinguishes, at
+    n = 128, dz = 1, 12 slices: empty slices (0, 1 and 11), an element
+    absent throughout (O), planes of 1, 2, 8 atoms (direct 2-D DFT) and of 9
+    and tens to hundreds of atoms (row DFTs + column transform).  Coordinates are
+    (z, x, y); the bounding box is centred by two H atoms at its corners."""
+    rng = np.random.default_rng(seed)
+
+    def cloud(n, z0, z1):
+        z = rng.uniform(z0 + 0.05, z1 - 0.05, n)
+        xy = rng.uniform(-39.0, 39.0, (n, 2))
+        return np.column_stack([z, xy])
+
+    parts = [(1, np.array([[-4.0, -40.0, -40.0], [4.0, 40.0, 40.0]])),
+             (1, cloud(300, -4.0, 4.0)),
+             (6, cloud(600, -4.0, -1.0)),
+             (7, cloud(8, 0.0, 1.0)), (7, cloud(9, 1.0, 2.0)),
+             (16, cloud(1, -1.0, 0.0)), (16, cloud(2, 2.0, 3.0))]
+    el = np.concatenate([np.full(len(c), z, np.int64) for z, c in parts])
+    xyz = np.concatenate([c for _, c in parts])
+    return atm.AtomList(elements=el, coordinates=xyz, label="sparse planes")
+
+
+def test_sparse_element_planes_match_oracle():
+    tem = _tem()
+    n, ns, dz, pixel = 128, 12, 1.0, 1.0
+    mol = _sparse_planes_molecule()
+    pipe = BatchPipe(tem, pixel, dz, n, ns, ELEMS)
+    batch = pipe.upload([pipe.prepare(mol)])
+    images = pipe.run(batch).cpu().numpy()
+    waves, slices = pipe.exit_waves_and_slices(batch)
+    i32, i64, x32, x64 = _oracle_image(mol, tem, n, ns, dz, pixel)
+    ei, ex = orc.rel_l2(images[0], i32), orc.rel_l2(waves[0].cpu().numpy(), x32)
+    ei64, ex64 = orc.rel_l2(images[0], i64), orc.rel_l2(waves[0].cpu().numpy(), x64)
+    ref = pipe.build_slices(batch).cpu().numpy()
+    got = slices.cpu().numpy()
+    es = orc.rel_l2(got, ref)
+    print(f"[sparse planes] image {ei:.2e} (fp64 {ei64:.2e}), exit wave {ex:.2e} "
+          f"(fp64 {ex64:.2e}), slices vs builder {es:.2e}", flush=True)
+    assert ei < TOL and ei64 < TOL and ex < TOL and ex64 < TOL and es < 2e-6
+    # empty slices are the clip value exactly; each slice with atoms matches too
+    for s in (0, 1, 11):
+        assert np.all(got[0, s] == np.float32(1e-13))
+    for s in range(2, 11):
+        assert orc.rel_l2(got[0, s], ref[0, s]) < 2e-6
+
+
 def test_fused_equals_unfused_and_is_deterministic():
     tem = _tem()
     mols = _mols([3000, 6000], 41)
