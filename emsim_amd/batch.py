@@ -322,3 +322,123 @@ class BatchPipe:
         mic = self.microscope
         return self.propagator.lens_intensity(waves, mic.cs_mm, mic.defocus, mic.aperture,
                                               overwrite_wave=True)
+
+
+def batched(items, size: int):
+    """Consecutive lists of ``size`` items (the last may be shorter)."""
+    if size < 1:
+        raise ValueError("batch size must be >= 1")
+    chunk = []
+    for item in items:
+        chunk.append(item)
+        if len(chunk) == size:
+            yield chunk
+            chunk = []
+    if chunk:
+        yield chunk
+
+
+class BatchSim:
+    """Device-resident form of the reference's simulation loop
+    (``emsim/simulator.py:21-91``).
+
+    A host thread prepares the molecules of ``mols`` (centre, sort by
+    element) and groups them into batches of ``batch``; the calling thread
+    uploads each batch once and runs ``BatchPipe.run`` on the current stream.
+    Each batch's images are copied to the host on a second stream (default
+    priority) into a ring of ``depth`` pinned buffers, so the copy of batch
+    k and the ``result_handler`` calls for the batch before it overlap the
+    simulation of batch k + 1.  ``result_handler(image, label)`` receives a
+    float32 numpy array of its own per molecule, in input order.  As in the
+    reference, a molecule (or batch) that fails is logged and skipped.
+    """
+
+    def __init__(self, pipe: "BatchPipe", mols, result_handler, batch: int = 64,
+                 depth: int = 2):
+        if depth < 1:
+            raise ValueError("depth must be >= 1")
+        self.pipe, self.mols, self.result_handler = pipe, mols, result_handler
+        self.batch, self.depth = int(batch), int(depth)
+        self.delivered = 0
+        self.failed = 0
+
+    def _producer(self, q, stop):
+        import logging
+        log = logging.getLogger(__name__)
+        try:
+            for chunk in batched(self.mols, self.batch):
+                prepared = []
+                for mol in chunk:
+                    try:
+                        prepared.append(self.pipe.prepare(mol))
+                    except Exception as e:   # reference: log the failed image, go on
+                        log.warning(f"molecule {getattr(mol, 'label', '')!r} skipped: {e}")
+                        self.failed += 1
+                if prepared:
+                    q.put(prepared)
+                if stop.is_set():
+                    break
+        except Exception as e:   # an iterator that raises ends the stream
+            log.warning(f"molecule iterator failed with {type(e)}: {e}")
+        q.put(None)
+
+    def run(self) -> int:
+        """Simulate every molecule; returns the number of images delivered."""
+        import logging
+        from collections import deque
+        from queue import Queue
+        from threading import Event, Thread
+        log = logging.getLogger(__name__)
+        n = self.pipe.n
+        q, stop = Queue(maxsize=2), Event()
+        producer = Thread(target=self._producer, args=(q, stop), daemon=True)
+        producer.start()
+        copy_stream = torch.cuda.Stream()
+        ring = [torch.empty((self.batch, n, n), dtype=torch.float32, pin_memory=True)
+                for _ in range(self.depth)]
+        pending = deque()   # (slot, copy event, device images kept alive, labels)
+        slot = 0
+
+        def deliver():
+            s, ev, _, labels = pending.popleft()
+            ev.synchronize()
+            host = ring[s].numpy()
+            for i, label in enumerate(labels):
+                self.result_handler(host[i].copy(), label)
+                self.delivered += 1
+
+        try:
+            while True:
+                prepared = q.get()
+                if prepared is None:
+                    break
+                labels = [m.label for m in prepared]
+                try:
+                    images = self.pipe.run(self.pipe.upload(prepared))
+                except Exception as e:
+                    log.warning(f"batch of {len(prepared)} molecules failed: {e}")
+                    self.failed += len(prepared)
+                    continue
+                done = torch.cuda.Event()
+                done.record()
+                if len(pending) == self.depth:
+                    deliver()   # frees the ring slot about to be reused
+                copy_stream.wait_event(done)
+                with torch.cuda.stream(copy_stream):
+                    ring[slot][:len(prepared)].copy_(images, non_blocking=True)
+                    copied = torch.cuda.Event()
+                    copied.record(copy_stream)
+                images.record_stream(copy_stream)
+                pending.append((slot, copied, images, labels))
+                slot = (slot + 1) % self.depth
+            while pending:
+                deliver()
+        finally:
+            stop.set()
+            while producer.is_alive():   # unblock a producer waiting on a full queue
+                try:
+                    q.get_nowait()
+                except Exception:
+                    pass
+                producer.join(timeout=0.05)
+        return self.delivered

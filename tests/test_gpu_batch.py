@@ -118,6 +118,28 @@ def test_sparse_element_planes_match_oracle():
         assert orc.rel_l2(got[0, s], ref[0, s]) < 2e-6
 
 
+def test_batch_sim_delivers_in_order_and_skips_failures():
+    """The batched simulation loop: batches of 3 over 7 molecules (the last
+    batch partial), images through the pinned ring equal to one fused run of
+    all molecules, input order and labels kept, a molecule with an element
+    outside the pipe's list logged and skipped."""
+    from emsim_amd.batch import BatchSim
+    tem = _tem()
+    mols = _mols([2000, 3000, 2500, 4000, 1500, 3500, 2200], 71)
+    for i, m in enumerate(mols):
+        m.label = f"mol{i}"
+    bad = atm.AtomList(elements=np.array([79, 6]), coordinates=np.zeros((2, 3)), label="gold")
+    pipe = BatchPipe(tem, 1.0, 2.0, 128, 8, ELEMS)
+    ref = pipe.run(pipe.upload([pipe.prepare(m) for m in mols])).cpu().numpy()
+    got = []
+    sim = BatchSim(pipe, mols[:3] + [bad] + mols[3:], lambda img, label: got.append((label, img)),
+                   batch=3, depth=2)
+    assert sim.run() == 7 and sim.failed == 1
+    assert [label for label, _ in got] == [f"mol{i}" for i in range(7)]
+    for i, (_, img) in enumerate(got):
+        np.testing.assert_array_equal(img, ref[i])
+
+
 def test_fused_equals_unfused_and_is_deterministic():
     tem = _tem()
     mols = _mols([3000, 6000], 41)

@@ -134,3 +134,12 @@ def test_gpu_modules_import_without_a_gpu():
                  "emsim_amd.train_fno", "emsim_amd.profiling", "emsim_amd.pipe",
                  "emsim_amd.simulator"):
         importlib.import_module(name)
+
+
+def test_batched_groups_in_order():
+    from emsim_amd.batch import batched
+    assert list(batched(range(7), 3)) == [[0, 1, 2], [3, 4, 5], [6]]
+    assert list(batched(range(6), 3)) == [[0, 1, 2], [3, 4, 5]]
+    assert list(batched([], 4)) == []
+    with pytest.raises(ValueError):
+        list(batched(range(3), 0))
