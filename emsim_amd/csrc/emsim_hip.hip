@@ -221,6 +221,12 @@ struct emsim_hip_slice_builder {
     const float2* tw = nullptr;
 };
 
+namespace {
+// Rows of the fused path's Y also hold the spectrum kernel's per-element
+// column-0 transforms in padding columns H .. H + n_elems - 1.
+int fused_ld(const emsim_hip_slice_builder* b) { return half_ld(b->H + b->n_elems); }
+}  // namespace
+
 struct emsim_hip_wave_propagator {
     int n1 = 0, n2 = 0;
     float d1 = 0, d2 = 0, lam = 0, gamma = 0;
@@ -1014,7 +1020,11 @@ int launch_spec_sparse(const uint32_t* row_start, const uint16_t* bucket, const 
                        int ldh, int n2, int bandlimit, int band_r2, float inv_npix, int batch,
                        const float2* tw, hipStream_t s) {
     using Cfg = SpecCfg<N>;
-    const dim3 grid((unsigned)(ceil_div(H, Cfg::C) * (int64_t)batch));
+    if (H != n2 / 2 + 1 || n2 < 2 || ldh < H + n_elems)
+        return fail(EMSIM_HIP_EINVAL, "spectrum: needs H = n2/2 + 1, n2 >= 2, ldh >= H + n_elems");
+    // with spec_pack(N), column n2/2 is carried by column 0
+    const int cols = spec_pack(N) ? n2 / 2 : H;
+    const dim3 grid((unsigned)(ceil_div(cols, Cfg::C) * (int64_t)batch));
     EMSIM_LAUNCH(N, (k_spec_sparse<N>), grid, dim3(Cfg::BLOCK), 0, s, row_start, bucket, twq, sf,
                  Y, slice, n_slices, n_elems, H, ldh, n2, batch, bandlimit, band_r2, inv_npix,
                  tw);
@@ -1089,7 +1099,7 @@ FusedWorkspace fused_layout(const emsim_hip_slice_builder* b, int batch, int64_t
     w.n_rows = (int64_t)batch * b->n_slices * b->n_elems * b->n1;
     const int64_t n_scan = w.n_rows + 1;
     const int64_t n_blocks = ceil_div(n_scan, kScanTile);
-    const int ldh = half_ld(b->H);
+    const int ldh = fused_ld(b);
     size_t off = 0;
     w.mol_off = (int64_t*)(base + off);    off += align_up(sizeof(int64_t) * (size_t)(batch + 1));
     w.elem_off = (int64_t*)(base + off);
@@ -1256,7 +1266,7 @@ int simulate_core(const emsim_hip_slice_builder* b, const emsim_hip_wave_propaga
 
     // 2. the slices
     float2* wave = mode == 0 ? ws.wave : (float2*)out;
-    const int ldh = half_ld(b->H);
+    const int ldh = fused_ld(b);
     const int r = (b->n1 < b->n2 ? b->n1 : b->n2) / 2;
     const float inv_npix = 1.0f / (float)((int64_t)b->n1 * b->n2);
     const int64_t npix = (int64_t)b->n1 * b->n2;

@@ -96,10 +96,20 @@ __global__ void k_csr_scatter(int64_t n_atoms, const int32_t* __restrict__ atom_
 // ---------------------------------------------------------------------------
 // When the (molecule, slice)'s row table (n_elems * N + 1 entries), its atom
 // y-bins and the Q30 twiddles (n2 entries) all fit, they are staged in LDS
-// once per block; otherwise the row DFTs read all three from global memory.
-constexpr int kSpecRowsLds = 4097;
+// once per block.  Otherwise (dense slices) the row DFTs go through the
+// bucket in pieces: the rows t + m T of one m are adjacent, so their atoms
+// are one contiguous bucket range, staged kSpecBucketLds entries at a time;
+// the twiddles stay in LDS whenever n2 fits.  Columns of 2048 and more never
+// fit the row table, so its LDS holds their longer twiddle table instead.
+// Column n2/2 is folded into column 0 for N >= 2048, where the lone
+// (n2/2 + 1)-th column would otherwise add a block round to single-image
+// launches (1025 columns in 2-column tiles on 256 CUs).
+constexpr bool spec_pack(int N) { return N >= 2048; }
+template <int N> struct SpecLds {
+    static constexpr int kRows = N <= 1024 ? 4097 : 1;
+    static constexpr int kTwq = N <= 1024 ? 1024 : N;
+};
 constexpr int kSpecBucketLds = 8192;
-constexpr int kSpecTwLds = 1024;
 // Element planes with at most this many atoms in the slice take the direct
 // 2-D DFT instead of row DFTs + a column transform.
 #ifndef EMSIM_SPEC_DIRECT
@@ -127,26 +137,84 @@ __device__ __forceinline__ unsigned spec_item(unsigned L, unsigned total) {
 // twiddles w^(col y) of the row's atoms summed in 64-bit integers.  rows
 // holds absolute bucket positions (minus base: the position of bkt[0]);
 // twiddle p sits at tws[p * tw_step].
+// A packed lane (column 0 carrying column n2/2) sums the real parts of
+// both columns' twiddles: 1 into sr and (-1)^y into si.
 template <int EPT, int T>
 __device__ __forceinline__ void row_dft(float2 (&v)[EPT], bool valid, int r0, int col, int mask,
                                         const uint32_t* __restrict__ rows, uint32_t base,
                                         const uint16_t* __restrict__ bkt,
-                                        const int2* __restrict__ tws, int tw_step) {
+                                        const int2* __restrict__ tws, int tw_step, bool packed) {
 #pragma unroll
     for (int m = 0; m < EPT; ++m) {
         int64_t sr = 0, si = 0;
         if (valid) {
             const int r = r0 + m * T;
             const uint32_t beg = rows[r] - base, end = rows[r + 1] - base;
-            for (uint32_t k = beg; k < end; ++k) {
-                const int p = (col * (int)bkt[k]) & mask;
-                const int2 w = tws[p * tw_step];
-                sr += w.x;
-                si += w.y;
+            if (!packed) {
+                for (uint32_t k = beg; k < end; ++k) {
+                    const int p = (col * (int)bkt[k]) & mask;
+                    const int2 w = tws[p * tw_step];
+                    sr += w.x;
+                    si += w.y;
+                }
+            } else {
+                const int w0 = tws[0].x, wh = tws[((mask + 1) >> 1) * tw_step].x;
+                for (uint32_t k = beg; k < end; ++k) {
+                    sr += w0;
+                    si += (bkt[k] & 1) ? wh : w0;
+                }
             }
         }
         v[m] = make_float2((float)sr * kQ30Inv, (float)si * kQ30Inv);
     }
+}
+
+// The same sums for dense slices: for each m, the bucket range of rows
+// r0 - t + m T .. + T (all T threads of the column group) is staged through
+// lds_bkt in pieces of kSpecBucketLds entries.  Rows come from global
+// memory (rows[r] absolute), twiddles from tws[p * tw_step] (LDS when
+// TWL).  Every thread of the block must call it (block barriers).
+template <int EPT, int T, bool TWL>
+__device__ __forceinline__ void row_dft_pieces(float2 (&v)[EPT], bool valid, int r0, int t,
+                                               int col, int mask,
+                                               const uint32_t* __restrict__ rows,
+                                               const uint16_t* __restrict__ bucket,
+                                               uint16_t* lds_bkt, const int2* tws, int tw_step,
+                                               int block, bool packed) {
+#pragma unroll
+    for (int m = 0; m < EPT; ++m) {
+        const int r = r0 + m * T;              // this thread's row
+        const int rg = r - t;                  // first row of the group
+        const uint32_t lo = __builtin_amdgcn_readfirstlane(rows[rg]);
+        const uint32_t hi = __builtin_amdgcn_readfirstlane(rows[rg + T]);
+        const uint32_t beg = rows[r], end = rows[r + 1];
+        int64_t sr = 0, si = 0;
+        for (uint32_t p0 = lo; p0 < hi; p0 += (uint32_t)kSpecBucketLds) {
+            const uint32_t p1 = hi - p0 > (uint32_t)kSpecBucketLds ? p0 + kSpecBucketLds : hi;
+            __syncthreads();   // the previous piece is consumed
+            for (uint32_t i = threadIdx.x; i < p1 - p0; i += block) lds_bkt[i] = bucket[p0 + i];
+            __syncthreads();
+            if (valid) {
+                const uint32_t kb = beg > p0 ? beg : p0, ke = end < p1 ? end : p1;
+                if (!packed) {
+                    for (uint32_t k = kb; k < ke; ++k) {
+                        const int p = (col * (int)lds_bkt[k - p0]) & mask;
+                        const int2 w = tws[p * tw_step];
+                        sr += w.x;
+                        si += w.y;
+                    }
+                } else {
+                    const int w0 = tws[0].x, wh = tws[((mask + 1) >> 1) * tw_step].x;
+                    for (uint32_t k = kb; k < ke; ++k) {
+                        sr += w0;
+                        si += (lds_bkt[k - p0] & 1) ? wh : w0;
+                    }
+                }
+            }
+        }
+        v[m] = make_float2((float)sr * kQ30Inv, (float)si * kQ30Inv);
+    }
+    __syncthreads();   // lds_bkt is free again
 }
 
 // Column transform of the spectrum kernel, twiddles from LDS or the table.
@@ -167,9 +235,9 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
     constexpr int EPT = Cfg::EPT;
     __shared__ float lds_re[Cfg::C * Cfg::STRIDE];
     __shared__ float lds_im[Cfg::C * Cfg::STRIDE];
-    __shared__ uint32_t lds_rows[kSpecRowsLds];
+    __shared__ uint32_t lds_rows[SpecLds<N>::kRows];
     __shared__ uint16_t lds_bucket[kSpecBucketLds];
-    __shared__ int2 lds_twq[kSpecTwLds];
+    __shared__ int2 lds_twq[SpecLds<N>::kTwq];
     __shared__ uint32_t lds_atoms_raw[kSpecDirect > 0 ? kSpecDirect : 1];   // (x << 16) | y
     __shared__ uint32_t lds_atoms[kSpecDirect > 0 ? kSpecDirect : 1];       // the same, sorted
     // the column transforms' twiddles (N <= 512; longer columns read the table,
@@ -182,19 +250,27 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
     const int c = threadIdx.x % Cfg::C;
     const int t = threadIdx.x / Cfg::C;
     const int col = tile * Cfg::C + c;
-    const bool valid = col < H;
+    // N >= 2048: column n2/2 rides with column 0 (spec_pack), so the tiles
+    // cover columns 0 .. n2/2 - 1 only
+    constexpr bool PACK = spec_pack(N);
+    const int half = n2 / 2;
+    const bool valid = col < (PACK ? half : H);
+    const bool packed = PACK && col == 0;
     float* re = lds_re + c * Cfg::STRIDE;
     float* im = lds_im + c * Cfg::STRIDE;
     const int tw_step = kTwN / n2;
     const int64_t sf_plane = (int64_t)N * H;
+    // the packed lane parks each element's transform Z_e in padding column
+    // H + e of its molecule's Y rows (ldh >= H + n_elems)
+    float2* zpark = Y + (int64_t)b * N * ldh + H;
 
     // stage this (molecule, slice)'s rows (rows of all elements are adjacent)
     const int64_t rbase = (((int64_t)b * n_slices + slice) * n_elems) * N;
     const int n_rows = n_elems * N;
-    const bool rows_lds = n_rows < kSpecRowsLds;
+    const bool rows_lds = n_rows < SpecLds<N>::kRows;
     if (rows_lds)
         for (int i = threadIdx.x; i <= n_rows; i += Cfg::BLOCK) lds_rows[i] = row_start[rbase + i];
-    const bool tw_lds = n2 <= kSpecTwLds;
+    const bool tw_lds = n2 <= SpecLds<N>::kTwq;
     if (tw_lds)
         for (int i = threadIdx.x; i < n2; i += Cfg::BLOCK) lds_twq[i] = twq[i * tw_step];
     if constexpr (kTwInLds) TwLds<N>::fill(lds_tw, tw, threadIdx.x, Cfg::BLOCK);
@@ -210,7 +286,7 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
         kend = __builtin_amdgcn_readfirstlane(row_start[rbase + n_rows]);
     }
     const uint32_t kcount = kend - kbase;
-    // everything in LDS (the usual case), or everything from global memory
+    // everything in LDS (the usual case), or the bucket in pieces
     const bool all_lds = rows_lds && tw_lds && kcount <= (uint32_t)kSpecBucketLds;
     if (all_lds)
         for (uint32_t i = threadIdx.x; i < kcount; i += Cfg::BLOCK) lds_bucket[i] = bucket[kbase + i];
@@ -232,7 +308,14 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
         }
         const uint32_t n_e = e_end - e_beg;
         // an element without atoms in this slice adds exactly zero
-        if (n_e == 0) continue;
+        if (n_e == 0) {
+            if (packed) {
+#pragma unroll
+                for (int m = 0; m < EPT; ++m)
+                    zpark[(int64_t)(t + m * Cfg::T) * ldh + e] = make_float2(0.f, 0.f);
+            }
+            continue;
+        }
         any = true;
         // form factors first: their loads stay in flight behind the transform
         const float* f = sf + (int64_t)e * sf_plane + col;
@@ -273,7 +356,10 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
                 const uint32_t xy = lds_atoms[a];
                 const int x = (int)(xy >> 16), y = (int)(xy & 0xffffu);
                 const int2 q = lds_twq[(col * y) & (n2 - 1)];
-                const float2 wy = make_float2((float)q.x * kQ30Inv, (float)q.y * kQ30Inv);
+                float2 wy = make_float2((float)q.x * kQ30Inv, (float)q.y * kQ30Inv);
+                if (packed)   // column 0 (1) + i column n2/2 ((-1)^y), both real
+                    wy = make_float2((float)lds_twq[0].x * kQ30Inv,
+                                     (float)((y & 1) ? lds_twq[half].x : lds_twq[0].x) * kQ30Inv);
 #pragma unroll
                 for (int m = 0; m < EPT; ++m) {
                     const int p = ((t + m * Cfg::T) * x) & (N - 1);
@@ -289,10 +375,15 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
             // block-uniform choice, so each path's loads have one address space
             if (all_lds)
                 row_dft<EPT, Cfg::T>(v, valid, e * N + t, col, n2 - 1, lds_rows, kbase, lds_bucket,
-                                     lds_twq, 1);
+                                     lds_twq, 1, packed);
+            else if (tw_lds)
+                row_dft_pieces<EPT, Cfg::T, true>(v, valid, e * N + t, t, col, n2 - 1,
+                                                  row_start + rbase, bucket, lds_bucket, lds_twq,
+                                                  1, Cfg::BLOCK, packed);
             else
-                row_dft<EPT, Cfg::T>(v, valid, e * N + t, col, n2 - 1, row_start + rbase, 0u,
-                                     bucket, twq, tw_step);
+                row_dft_pieces<EPT, Cfg::T, false>(v, valid, e * N + t, t, col, n2 - 1,
+                                                   row_start + rbase, bucket, lds_bucket, twq,
+                                                   tw_step, Cfg::BLOCK, packed);
             spec_fft<N, EPT, false, kTwInLds>(v, re, im, t, lds_tw, tw);
         }
 #pragma unroll
@@ -300,13 +391,20 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
             acc[m].x += v[m].x * fe[m];
             acc[m].y += v[m].y * fe[m];
         }
+        if (packed) {
+#pragma unroll
+            for (int m = 0; m < EPT; ++m) zpark[(int64_t)(t + m * Cfg::T) * ldh + e] = v[m];
+        }
     }
     if (!any) {
         // empty slice of this molecule: the half spectrum is exactly zero
         if (valid) {
             float2* dst = Y + (int64_t)b * N * ldh + col;
 #pragma unroll
-            for (int m = 0; m < EPT; ++m) dst[(int64_t)(t + m * Cfg::T) * ldh] = make_float2(0.f, 0.f);
+            for (int m = 0; m < EPT; ++m) {
+                dst[(int64_t)(t + m * Cfg::T) * ldh] = make_float2(0.f, 0.f);
+                if (packed) dst[(int64_t)(t + m * Cfg::T) * ldh + half] = make_float2(0.f, 0.f);
+            }
         }
         return;
     }
@@ -321,11 +419,79 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
         }
         acc[m] = make_float2(acc[m].x * s, acc[m].y * s);
     }
+    if (PACK && tile == 0) {
+        // Column n2/2 from column 0's packed transforms Z_e = F0_e + i FN_e
+        // (both columns' row DFTs are real, the form factors and masks even
+        // in kx): P = sum_e sf(k, 0) Z_e (acc, already scaled) and
+        // Q = sum_e sf(k, n2/2) Z_e give A0 = (P(k) + conj P(-k)) / 2 and
+        // AN = (Q(k) - conj Q(-k)) / 2i; one inverse transform of A0 + i AN
+        // returns both real columns.  Block-uniform branch: every thread
+        // reaches the barriers.
+        using L = LdsLayout<N>;
+        float* re1 = lds_re + Cfg::STRIDE;   // column 1's planes, free here
+        float* im1 = lds_im + Cfg::STRIDE;
+        if (packed) {
+            float2 qa[EPT];
+#pragma unroll
+            for (int m = 0; m < EPT; ++m) qa[m] = make_float2(0.f, 0.f);
+            for (int e = 0; e < n_elems; ++e) {   // element-outer: EPT loads in flight
+                const float* fh = sf + (int64_t)e * sf_plane + half;
+#pragma unroll
+                for (int m = 0; m < EPT; ++m) {
+                    const int64_t k1 = t + m * Cfg::T;
+                    const float2 ze = zpark[k1 * ldh + e];
+                    const float fq = fh[k1 * H];
+                    qa[m].x += ze.x * fq;
+                    qa[m].y += ze.y * fq;
+                }
+            }
+            const int kh = freq_int(half, n2);
+#pragma unroll
+            for (int m = 0; m < EPT; ++m) {
+                const int k1 = t + m * Cfg::T;
+                float s = inv_npix;
+                if (bandlimit) {
+                    const int ki = freq_int(k1, N);
+                    if (ki * ki + kh * kh > band_r2) s = 0.f;
+                }
+                re[L::at(k1)] = acc[m].x;
+                im[L::at(k1)] = acc[m].y;
+                re1[L::at(k1)] = qa[m].x * s;
+                im1[L::at(k1)] = qa[m].y * s;
+            }
+        }
+        __syncthreads();
+        if (packed) {
+#pragma unroll
+            for (int m = 0; m < EPT; ++m) {
+                const int k1 = t + m * Cfg::T;
+                const int nk = (N - k1) & (N - 1);
+                const float2 p = acc[m];
+                const float2 q = make_float2(re1[L::at(k1)], im1[L::at(k1)]);
+                const float2 pm = make_float2(re[L::at(nk)], im[L::at(nk)]);
+                const float2 qm = make_float2(re1[L::at(nk)], im1[L::at(nk)]);
+                const float2 a0 = make_float2(0.5f * (p.x + pm.x), 0.5f * (p.y - pm.y));
+                const float2 an = make_float2(0.5f * (q.y + qm.y), -0.5f * (q.x - qm.x));
+                acc[m] = make_float2(a0.x - an.y, a0.y + an.x);
+            }
+        }
+        __syncthreads();
+    }
     spec_fft<N, EPT, true, kTwInLds>(acc, re, im, t, lds_tw, tw);
     if (valid) {
         float2* dst = Y + (int64_t)b * N * ldh + col;
+        if (!packed) {
 #pragma unroll
-        for (int m = 0; m < EPT; ++m) dst[(int64_t)(t + m * Cfg::T) * ldh] = acc[m];
+            for (int m = 0; m < EPT; ++m) dst[(int64_t)(t + m * Cfg::T) * ldh] = acc[m];
+        } else {
+            // y0 + i yN: the imaginary parts of columns 0 and n2/2 are zero
+#pragma unroll
+            for (int m = 0; m < EPT; ++m) {
+                const int64_t o = (int64_t)(t + m * Cfg::T) * ldh;
+                dst[o] = make_float2(acc[m].x, 0.f);
+                dst[o + half] = make_float2(acc[m].y, 0.f);
+            }
+        }
     }
 }
 

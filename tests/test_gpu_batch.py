@@ -118,6 +118,30 @@ def test_sparse_element_planes_match_oracle():
         assert orc.rel_l2(got[0, s], ref[0, s]) < 2e-6
 
 
+def test_dense_slices_match_oracle():
+    """Slices with more atoms than the spectrum kernel stages at once
+    (> 8192 per molecule and slice): the row DFTs take the bucket in pieces.
+    A 256^2 random slab, 40k atoms in 4 slices of 2 A (about 10k each)."""
+    tem = _tem()
+    rng = np.random.default_rng(5)
+    n_atoms = 40000
+    zs = rng.choice([1, 6, 8], size=n_atoms, p=[0.6, 0.2, 0.2])
+    xyz = np.column_stack([rng.uniform(-3.95, 3.95, n_atoms),
+                           rng.uniform(-120.0, 120.0, (n_atoms, 2))])
+    mol = atm.AtomList(elements=zs, coordinates=xyz, label="dense slab")
+    n, ns, dz, pixel = 256, 4, 2.0, 1.0
+    pipe = BatchPipe(tem, pixel, dz, n, ns, ELEMS)
+    batch = pipe.upload([pipe.prepare(mol)])
+    images = pipe.run(batch).cpu().numpy()
+    waves, slices = pipe.exit_waves_and_slices(batch)
+    i32, i64, x32, x64 = _oracle_image(mol, tem, n, ns, dz, pixel)
+    ei, ex = orc.rel_l2(images[0], i32), orc.rel_l2(waves[0].cpu().numpy(), x32)
+    es = orc.rel_l2(slices.cpu().numpy(), pipe.build_slices(batch).cpu().numpy())
+    print(f"[dense slices] image {ei:.2e}, exit wave {ex:.2e}, slices vs builder {es:.2e}",
+          flush=True)
+    assert ei < TOL and ex < TOL and es < 2e-6
+
+
 def test_batch_sim_delivers_in_order_and_skips_failures():
     """The batched simulation loop: batches of 3 over 7 molecules (the last
     batch partial), images through the pinned ring equal to one fused run of
