@@ -96,15 +96,17 @@ def learned_bench(args, cfg, tem, mols, plain_ips):
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     # kernel table from a second, event-instrumented pass of the same steps
-    hip_ext.timing_collect()
-    hip_ext.timing_enable(True)
-    t1 = time.perf_counter()
-    for _ in range(args.steps):
-        lp.run(batch)
-    torch.cuda.synchronize()
-    elapsed_instr = time.perf_counter() - t1
-    hip_ext.timing_enable(False)
-    kernels = hip_ext.timing_collect()
+    kernels, elapsed_instr = [], None
+    if not args.no_timing:
+        hip_ext.timing_collect()
+        hip_ext.timing_enable(True)
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            lp.run(batch)
+        torch.cuda.synchronize()
+        elapsed_instr = time.perf_counter() - t1
+        hip_ext.timing_enable(False)
+        kernels = hip_ext.timing_collect()
     assert torch.isfinite(img).all().item()
     value = args.batch * args.steps / elapsed
     # the same with the sampled fine-slice check on every 4th batch only
@@ -156,7 +158,7 @@ def learned_bench(args, cfg, tem, mols, plain_ips):
                 "achieved_tflops": None if tflops is None else round(tflops, 1),
                 "mfma_peak_tflops": 2500.0,
                 "mfma_frac": None if tflops is None else round(tflops / 2500.0, 4)},
-        "gpu_busy_frac": round(gpu_total * 1e-3 / elapsed_instr, 4),
+        "gpu_busy_frac": round(gpu_total * 1e-3 / elapsed_instr, 4) if kernels else None,
         "kernels": [{"kernel": k["name"], "launches": k["count"],
                      "mean_us": round(k["mean_ms"] * 1e3, 2),
                      "ms_per_step": round(k["total_ms"] / args.steps, 3)} for k in top],
