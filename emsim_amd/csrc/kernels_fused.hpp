@@ -105,11 +105,14 @@ __global__ void k_csr_scatter(int64_t n_atoms, const int32_t* __restrict__ atom_
 // (n2/2 + 1)-th column would otherwise add a block round to single-image
 // launches (1025 columns in 2-column tiles on 256 CUs).
 constexpr bool spec_pack(int N) { return N >= 2048; }
+constexpr int kSpecBucketLds = 8192;
 template <int N> struct SpecLds {
     static constexpr int kRows = N <= 1024 ? 4097 : 1;
     static constexpr int kTwq = N <= 1024 ? 1024 : N;
+    // bucket entries staged at once; N >= 2048 runs one block per CU, which
+    // leaves it most of the CU's LDS
+    static constexpr int kBucket = N <= 1024 ? kSpecBucketLds : (N == 2048 ? 49152 : 24576);
 };
-constexpr int kSpecBucketLds = 8192;
 // Element planes with at most this many atoms in the slice take the direct
 // 2-D DFT instead of row DFTs + a column transform.
 #ifndef EMSIM_SPEC_DIRECT
@@ -217,6 +220,67 @@ __device__ __forceinline__ void row_dft_pieces(float2 (&v)[EPT], bool valid, int
     __syncthreads();   // lds_bkt is free again
 }
 
+// Dense slices at N >= 2048: the element's whole bucket range
+// [e_beg, e_end) is staged in pieces of CAP entries, and every thread
+// walks its EPT rows' atoms in each piece (per-row 64-bit accumulators), so
+// a piece costs one staging and two barriers however its atoms spread over
+// the rows.  Rows come from global memory (absolute positions).
+template <int EPT, int T, int CAP, bool TWL>
+__device__ __forceinline__ void row_dft_big(float2 (&v)[EPT], bool valid, int r0, int col,
+                                            int mask, const uint32_t* __restrict__ rows,
+                                            uint32_t e_beg, uint32_t e_end,
+                                            const uint16_t* __restrict__ bucket,
+                                            uint16_t* lds_bkt, const int2* tws, int tw_step,
+                                            int block, bool packed) {
+    int64_t sr[EPT], si[EPT];
+    uint32_t beg[EPT], end[EPT];
+#pragma unroll
+    for (int m = 0; m < EPT; ++m) {
+        sr[m] = 0;
+        si[m] = 0;
+        beg[m] = rows[r0 + m * T];
+        end[m] = rows[r0 + m * T + 1];
+    }
+    const int w0 = tws[0].x, wh = tws[((mask + 1) >> 1) * tw_step].x;
+    for (uint32_t p0 = e_beg; p0 < e_end; p0 += (uint32_t)CAP) {
+        const uint32_t p1 = e_end - p0 > (uint32_t)CAP ? p0 + CAP : e_end;
+        __syncthreads();   // the previous piece is consumed
+        for (uint32_t i = threadIdx.x; i < p1 - p0; i += block) lds_bkt[i] = bucket[p0 + i];
+        __syncthreads();
+        if (!valid) continue;
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) {
+            const uint32_t kb = beg[m] > p0 ? beg[m] : p0, ke = end[m] < p1 ? end[m] : p1;
+            uint32_t k = kb;
+            if (!packed) {
+                // four atoms per step: their loads are in flight together
+                for (; k + 4 <= ke; k += 4) {
+                    const uint16_t* yb = lds_bkt + (k - p0);
+                    const int2 a0 = tws[((col * (int)yb[0]) & mask) * tw_step];
+                    const int2 a1 = tws[((col * (int)yb[1]) & mask) * tw_step];
+                    const int2 a2 = tws[((col * (int)yb[2]) & mask) * tw_step];
+                    const int2 a3 = tws[((col * (int)yb[3]) & mask) * tw_step];
+                    sr[m] += ((int64_t)a0.x + a1.x) + ((int64_t)a2.x + a3.x);
+                    si[m] += ((int64_t)a0.y + a1.y) + ((int64_t)a2.y + a3.y);
+                }
+                for (; k < ke; ++k) {
+                    const int2 a = tws[((col * (int)lds_bkt[k - p0]) & mask) * tw_step];
+                    sr[m] += a.x;
+                    si[m] += a.y;
+                }
+            } else {
+                for (; k < ke; ++k) {
+                    sr[m] += w0;
+                    si[m] += (lds_bkt[k - p0] & 1) ? wh : w0;
+                }
+            }
+        }
+    }
+    __syncthreads();   // lds_bkt is free again
+#pragma unroll
+    for (int m = 0; m < EPT; ++m) v[m] = make_float2((float)sr[m] * kQ30Inv, (float)si[m] * kQ30Inv);
+}
+
 // Column transform of the spectrum kernel, twiddles from LDS or the table.
 template <int N, int EPT, bool INV, bool LDS>
 __device__ __forceinline__ void spec_fft(float2 (&v)[EPT], float* re, float* im, int t,
@@ -236,7 +300,7 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
     __shared__ float lds_re[Cfg::C * Cfg::STRIDE];
     __shared__ float lds_im[Cfg::C * Cfg::STRIDE];
     __shared__ uint32_t lds_rows[SpecLds<N>::kRows];
-    __shared__ uint16_t lds_bucket[kSpecBucketLds];
+    __shared__ uint16_t lds_bucket[SpecLds<N>::kBucket];
     __shared__ int2 lds_twq[SpecLds<N>::kTwq];
     __shared__ uint32_t lds_atoms_raw[kSpecDirect > 0 ? kSpecDirect : 1];   // (x << 16) | y
     __shared__ uint32_t lds_atoms[kSpecDirect > 0 ? kSpecDirect : 1];       // the same, sorted
@@ -287,7 +351,7 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
     }
     const uint32_t kcount = kend - kbase;
     // everything in LDS (the usual case), or the bucket in pieces
-    const bool all_lds = rows_lds && tw_lds && kcount <= (uint32_t)kSpecBucketLds;
+    const bool all_lds = rows_lds && tw_lds && kcount <= (uint32_t)SpecLds<N>::kBucket;
     if (all_lds)
         for (uint32_t i = threadIdx.x; i < kcount; i += Cfg::BLOCK) lds_bucket[i] = bucket[kbase + i];
     __syncthreads();
@@ -376,7 +440,16 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
             if (all_lds)
                 row_dft<EPT, Cfg::T>(v, valid, e * N + t, col, n2 - 1, lds_rows, kbase, lds_bucket,
                                      lds_twq, 1, packed);
-            else if (tw_lds)
+            else if constexpr (N >= 2048) {
+                if (tw_lds)
+                    row_dft_big<EPT, Cfg::T, SpecLds<N>::kBucket, true>(
+                        v, valid, e * N + t, col, n2 - 1, row_start + rbase, e_beg, e_end, bucket,
+                        lds_bucket, lds_twq, 1, Cfg::BLOCK, packed);
+                else
+                    row_dft_big<EPT, Cfg::T, SpecLds<N>::kBucket, false>(
+                        v, valid, e * N + t, col, n2 - 1, row_start + rbase, e_beg, e_end, bucket,
+                        lds_bucket, twq, tw_step, Cfg::BLOCK, packed);
+            } else if (tw_lds)
                 row_dft_pieces<EPT, Cfg::T, true>(v, valid, e * N + t, t, col, n2 - 1,
                                                   row_start + rbase, bucket, lds_bucket, lds_twq,
                                                   1, Cfg::BLOCK, packed);
