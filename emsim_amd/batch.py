@@ -16,6 +16,7 @@ last slice.  ``run_unfused`` is the same computation through the
 reference-shaped steps (dense histogram, ``make_multi_slices``,
 ``multislice_propagate``, ``lens_intensity``), kept as a cross-check.
 """
+import os
 from ctypes import POINTER, c_int64
 from dataclasses import dataclass
 from typing import List, Sequence, Tuple
@@ -29,6 +30,21 @@ from .backend._device import as_device, device, ptr, stream_handle
 from .backend.slice_builder_hip import MultiSlicesBuilder
 from .backend.wave_propagator_hip import WavePropagator
 from .tem import TEM
+
+
+def pool_map(fn, items, workers: int = None) -> list:
+    """[fn(x) for x in items] on a thread pool (numpy releases the GIL in
+    the sorts, reductions and copies of the host preparation); at most 16
+    threads, the CPU share a GPU process is sized for."""
+    items = list(items)
+    if workers is None:
+        workers = min(16, os.cpu_count() or 1)
+    workers = max(1, min(int(workers), len(items)))
+    if workers == 1:
+        return [fn(x) for x in items]
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(workers) as ex:
+        return list(ex.map(fn, items))
 
 
 @dataclass
@@ -75,7 +91,9 @@ class MixedBatch:
         return self._pipes[key]
 
     def slice_count(self, mols: Sequence[atm.AtomList], n: int) -> int:
-        thickest = max(float(m.space[0]) for m in mols)
+        return self._slices_for(max(float(m.space[0]) for m in mols), n)
+
+    def _slices_for(self, thickest: float, n: int) -> int:
         ns, _, _ = atm.determine_box_size((thickest, 1.0, 1.0),
                                           voxel_size=(self.dz, self.pixel_size, self.pixel_size),
                                           box_size=(None, n, n))
@@ -86,12 +104,14 @@ class MixedBatch:
         complex64 exit wave (mode 1) per item, on the GPU, in input order."""
         out: List[torch.Tensor] = [None] * len(items)
         self.last_groups = []
+        # host preparation of every molecule on a thread pool
+        cen = pool_map(atm.centralize, [m for m, _ in items])
+        thick = pool_map(lambda m: float(m.space[0]), cen)
         for n in sorted({int(sz) for _, sz in items}):
             idx = [i for i, (_, sz) in enumerate(items) if int(sz) == n]
-            mols = [atm.centralize(items[i][0]) for i in idx]
-            ns = self.slice_count(mols, n)
+            ns = self._slices_for(max(thick[i] for i in idx), n)
             p = self.pipe(n, ns)
-            batch = p.upload([p.prepare(m) for m in mols])
+            batch = p.upload(p.prepare_many([cen[i] for i in idx]))
             res = p._simulate(batch, mode)
             for j, i in enumerate(idx):
                 out[i] = res[j]
@@ -139,6 +159,11 @@ class BatchPipe:
         mol, zs, counts = atm.sort_elements_and_count(mol, self.elements)
         return PreparedMolecule(np.ascontiguousarray(mol.coordinates, dtype=np.float32),
                                 np.asarray(counts, dtype=np.int64), mol.label)
+
+    def prepare_many(self, mols: Sequence[atm.AtomList], workers: int = None) \
+            -> List[PreparedMolecule]:
+        """``prepare`` of every molecule, on a thread pool (``pool_map``)."""
+        return pool_map(self.prepare, mols, workers)
 
     def upload(self, mols: Sequence[PreparedMolecule]) -> PreparedBatch:
         """Concatenate prepared molecules and copy their atoms to the GPU once."""
@@ -366,14 +391,20 @@ class BatchSim:
         import logging
         log = logging.getLogger(__name__)
         try:
+            def attempt(mol):
+                try:
+                    return self.pipe.prepare(mol)
+                except Exception as e:   # reference: log the failed image, go on
+                    return e
+
             for chunk in batched(self.mols, self.batch):
                 prepared = []
-                for mol in chunk:
-                    try:
-                        prepared.append(self.pipe.prepare(mol))
-                    except Exception as e:   # reference: log the failed image, go on
-                        log.warning(f"molecule {getattr(mol, 'label', '')!r} skipped: {e}")
+                for mol, res in zip(chunk, pool_map(attempt, chunk)):
+                    if isinstance(res, Exception):
+                        log.warning(f"molecule {getattr(mol, 'label', '')!r} skipped: {res}")
                         self.failed += 1
+                    else:
+                        prepared.append(res)
                 if prepared:
                     q.put(prepared)
                 if stop.is_set():

@@ -143,3 +143,17 @@ def test_batched_groups_in_order():
     assert list(batched([], 4)) == []
     with pytest.raises(ValueError):
         list(batched(range(3), 0))
+
+
+def test_pool_map_keeps_order_and_raises():
+    from emsim_amd.batch import pool_map
+    assert pool_map(lambda x: x * x, range(50), workers=8) == [x * x for x in range(50)]
+    assert pool_map(lambda x: x + 1, [3], workers=8) == [4]
+    assert pool_map(lambda x: x, [], workers=4) == []
+
+    def bad(x):
+        if x == 7:
+            raise ValueError("seven")
+        return x
+    with pytest.raises(ValueError):
+        pool_map(bad, range(10), workers=4)
