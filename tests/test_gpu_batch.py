@@ -94,6 +94,27 @@ def _sparse_planes_molecule(seed=7):
     return atm.AtomList(elements=el, coordinates=xyz, label="sparse planes")
 
 
+@pytest.mark.parametrize("n,ns,dz,pixel", [(64, 3, 12.0, 1.0), (1024, 3, 12.0, 0.5),
+                                           (4096, 1, 36.0, 0.25)])
+def test_fused_other_sizes_match_oracle(n, ns, dz, pixel):
+    """Sizes whose spectrum pass takes other code: 1024 (row table beyond
+    the LDS stage: bucket pieces), 4096 (large pieces, column n2/2 folded
+    into column 0, 16 values per thread), 64 (one block holds several
+    tiles' worth of rows)."""
+    tem = _tem()
+    mol = _mols([3000], 81)[0]
+    pipe = BatchPipe(tem, pixel, dz, n, ns, ELEMS)
+    batch = pipe.upload([pipe.prepare(mol)])
+    image = pipe.run(batch).cpu().numpy()[0]
+    wave = pipe.exit_waves(batch).cpu().numpy()[0]
+    i32, i64, x32, x64 = _oracle_image(mol, tem, n, ns, dz, pixel)
+    ei, ex = orc.rel_l2(image, i32), orc.rel_l2(wave, x32)
+    ei64, ex64 = orc.rel_l2(image, i64), orc.rel_l2(wave, x64)
+    print(f"[fused n={n}] image {ei:.2e} (fp64 {ei64:.2e}), exit wave {ex:.2e} "
+          f"(fp64 {ex64:.2e})", flush=True)
+    assert ei < TOL and ei64 < TOL and ex < TOL and ex64 < TOL
+
+
 def test_sparse_element_planes_match_oracle():
     tem = _tem()
     n, ns, dz, pixel = 128, 12, 1.0, 1.0
