@@ -106,9 +106,12 @@ __global__ void k_csr_scatter(int64_t n_atoms, const int32_t* __restrict__ atom_
 // launches (1025 columns in 2-column tiles on 256 CUs).
 constexpr bool spec_pack(int N) { return N >= 2048; }
 constexpr int kSpecBucketLds = 8192;
+// Tables are sized by N (row table for up to 8 elements, twiddles for
+// n2 = N), so small transforms keep more blocks per CU; larger element counts
+// or n2 > N take the piece path.
 template <int N> struct SpecLds {
-    static constexpr int kRows = N <= 1024 ? 4097 : 1;
-    static constexpr int kTwq = N <= 1024 ? 1024 : N;
+    static constexpr int kRows = N <= 1024 ? (8 * N + 1 < 4097 ? 8 * N + 1 : 4097) : 1;
+    static constexpr int kTwq = N;
     // bucket entries staged at once; N >= 2048 runs one block per CU, which
     // leaves it most of the CU's LDS
     static constexpr int kBucket = N <= 1024 ? kSpecBucketLds : (N == 2048 ? 49152 : 24576);
