@@ -301,7 +301,7 @@ def cpu_baseline(mols, cfg, tem, budget_s: float = 12.0, max_images: int = 32):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: this benchmark runs on one GPU; --gpus {args.gpus} is not supported "
               f"(images would shard across GPUs with no collective)", file=sys.stderr)
         sys.exit(2)
     if args.calibrate_copy:

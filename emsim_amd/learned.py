@@ -1,7 +1,8 @@
 """Learned coarse-slice correction on the HIP backend.
 
 The reference has no learned component; this module adds one beside the
-plain path and keeps the plain path as its fallback and judge.
+plain path and keeps the plain path as its fallback and as the measure of its
+error.
 
 ``FNO`` wraps the library's Fourier neural operator (include/emsim_hip.h,
 ``emsim_hip_fno_*``): 4 spectral layers of 32 channels keeping 16 modes per
@@ -346,7 +347,7 @@ class GateReport:
 class LearnedPipe:
     """Coarse multislice + FNO correction, with plain fine-slice multislice as
     the fallback (taken when a norm-conservation or sampled-image residual
-    check fails) and as the judge of the error."""
+    check fails) and as the measure of the error."""
 
     def __init__(self, fno: FNO, pairs: CoarseFinePairs, norm_tol: float = 0.01,
                  check_tol: float = 0.01, check_every: int = 1):

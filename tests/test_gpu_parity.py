@@ -16,7 +16,7 @@ from tests._chain import pipe_oracle, slices_oracle
 
 pytestmark = pytest.mark.gpu
 
-TOL = 1e-5          # relative L2 vs the fftw-semantics oracle
+TOL = 1e-5          # relative L2 vs the fftw-semantics oracle (the parity bar)
 TOL64 = 1e-5        # relative L2 vs the FP64 oracle
 LAM, GAMMA = orc.wavelength_and_gamma(200)
 
