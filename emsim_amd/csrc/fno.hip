@@ -93,7 +93,7 @@ struct FnoWs {
     float* ty;           // (B n, 2 M2 C)
     float* pq[8];        // (B, 2 M1, 2 M2 C) per layer (training keeps all)
     float* ob;           // (B, 2 M1, M2 C)
-    float* u;            // (B, 2 n, M2 C)
+    float* u;            // (B, 2 n, M2 C); bf16 in the fused forward
     // backward
     float* dv;           // (P, C)
     float* dz;           // (P, C)
@@ -358,8 +358,11 @@ bool fused_rows_ok(const emsim_hip_fno* f) {
 }
 
 // x-transform of T, per-mode mix, inverse x-transform -> U (one layer)
+// x-transform, channel mix and inverse x-transform of layer l.  u16: U is
+// written as bf16, what the fused row kernel reads (it would round float32
+// U to bf16 the same way when staging it).
 int spectral_middle(const emsim_hip_fno* f, const FnoWs& w, int batch, int l, float* pq,
-                    hipStream_t s) {
+                    hipStream_t s, bool u16 = false) {
     const int n = f->n, C = f->C, M1 = f->M1, M2 = f->M2;
     int rc;
     emsim_gemm_desc d = gemm_desc();
@@ -379,7 +382,7 @@ int spectral_middle(const emsim_hip_fno* f, const FnoWs& w, int batch, int l, fl
     d.m = 2 * n; d.n = M2 * C; d.k = 2 * M1; d.batch = batch;
     d.a = f->gx; d.a_type = EMSIM_DT_BF16; d.a_sm = 2 * M1; d.a_sk = 1;
     d.b = w.ob; d.b_type = EMSIM_DT_F32; d.b_bs = (int64_t)2 * M1 * M2 * C; d.b_sk = M2 * C; d.b_sn = 1;
-    d.c = w.u; d.c_type = EMSIM_DT_F32; d.c_bs = (int64_t)2 * n * M2 * C; d.c_sm = M2 * C; d.c_sn = 1;
+    d.c = w.u; d.c_type = u16 ? EMSIM_DT_BF16 : EMSIM_DT_F32; d.c_bs = (int64_t)2 * n * M2 * C; d.c_sm = M2 * C; d.c_sn = 1;
     return launch_gemm(d, s);
 }
 
@@ -467,12 +470,13 @@ int forward_fused(const emsim_hip_fno* f, const FnoWs& w, const void* x, float* 
     if (rc) return rc;
     for (int l = 0; l < f->L; ++l) {
         const bool last = l + 1 == f->L;
-        if ((rc = spectral_middle(f, w, batch, l, training ? w.pq[l] : w.pq[0], s))) return rc;
+        if ((rc = spectral_middle(f, w, batch, l, training ? w.pq[l] : w.pq[0], s, true)))
+            return rc;
         a.vin = training ? w.v[l] : w.v[l & 1];
         a.k1 = f->C;
         a.w16 = f->params16 + f->off_w[l];
         a.bias = f->params + f->off_b[l];
-        a.U = w.u;
+        a.U = (const uint16_t*)w.u;
         a.act = last ? 0 : 1;
         a.vout = training ? w.v[l + 1] : (last ? nullptr : w.v[(l + 1) & 1]);
         a.zout = training ? w.z[l] : nullptr;

@@ -184,7 +184,7 @@ struct FnoRowArgs {
     int k1;                // input channels (C, or cin_pad for the lifting)
     const uint16_t* w16;   // (k1, 32) bf16, [k][c]
     const float* bias;     // (32)
-    const float* U;        // (rows, 2 M2, 32) float32: Uhat of every row, or null
+    const uint16_t* U;     // (rows, 2 M2, 32) bf16: Uhat of every row, or null
     const uint16_t* gy;    // (n, 2 M2) bf16
     int m2x2;              // 2 M2
     int act;               // GELU after the sum
@@ -296,10 +296,10 @@ __global__ __launch_bounds__(256) void k_fno_rows(FnoRowArgs a) {
         wT[c * kFnoWStride + k] = a.w16[i];
     }
     if (HAS_U) {
-        const float* u = a.U + row * m2x2 * 32;
+        const uint16_t* u = a.U + row * m2x2 * 32;
         for (int i = threadIdx.x; i < m2x2 * 32; i += 256) {
             const int j = i >> 5, c = i & 31;
-            uT[c * kFnoWStride + j] = f32_to_bf16(u[i]);
+            uT[c * kFnoWStride + j] = u[i];
         }
     }
     if (threadIdx.x < 32) lds_bias[threadIdx.x] = a.bias[threadIdx.x];

@@ -201,9 +201,11 @@ int emsim_hip_simulate(const emsim_hip_slice_builder* b, const emsim_hip_wave_pr
 /* One batched, strided GEMM, C = A B (+ A2 B2) (+ bias, GELU):
  * element (b, i, k) of A is at a + b a_bs + i a_sm + k a_sk (in elements of
  * a_type), likewise B (b, k, j), C (b, i, j).  k2 > 0 adds a second operand
- * pair summed over k2.  split_k > 0: the k range is cut into chunks of
- * split_k and chunk s writes its float32 partial sum to c + s c_split_stride
- * (batch must be 1; no bias / activation).  beta != 0 adds beta * C_old.
+ * pair summed over k2.  split_k > 0: the k range is cut into
+ * S = ceil(k / split_k) chunks and chunk s of batch entry b writes its
+ * float32 partial sum to c + (b S + s) c_split_stride (c_bs is unused; c
+ * must hold batch S partials; no bias / activation / beta).  beta != 0 adds
+ * beta * C_old.
  * z (nullable) receives the pre-activation in C's layout.  Operands are
  * rounded to bf16 on the way into the MFMA. */
 typedef struct emsim_gemm_desc {
