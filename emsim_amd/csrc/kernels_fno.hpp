@@ -226,27 +226,57 @@ constexpr int kRowU = 1, kRowAct = 2, kRowV = 4, kRowZ = 8, kRowT = 16, kRowProj
 
 // Lifting input channels 16 f + 8 h .. + 7 of pixel (row, y), built from the
 // exit wave and potentials exactly as k_fno_pack writes them.
-__device__ __forceinline__ bf16x8_t fno_packed_frag(const FnoRowArgs& a, int64_t row, int y, int f,
-                                                    int h) {
-    const int64_t b = row / a.n, x = row - b * a.n;
-    const int64_t npix = (int64_t)a.n * a.n;
-    float v[8];
+// Lifting input fragments of all four of a wave's tiles (K1F per tile),
+// built from the exit wave and potentials exactly as k_fno_pack writes them.
+// Every load is unconditional -- pixel and plane indices clamped into range,
+// values selected afterwards -- so all of them are in flight at once rather
+// than each waiting behind a lane-dependent or per-tile branch.
+template <int K1F>
+__device__ __forceinline__ void fno_packed_frags(const FnoRowArgs& a, int64_t row, int w, int l32,
+                                                 int h, bf16x8_t (&vf)[4][K1F]) {
+    const int n = a.n;
+    const int64_t b = row / n, x = row - b * n;
+    const int64_t npix = (int64_t)n * n;
+    // plane 0 of this row; with no slices, any in-bounds floats (masked below)
+    const float* srow = (a.ns > 0 ? a.slices + b * a.ns * npix : (const float*)a.wave) + x * n;
+    const float2* wrow = a.wave + row * n;
+    const int last = a.ns > 0 ? a.ns - 1 : 0;
+    float raw[4][K1F][8];
+    float2 wv[4];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        const int c = 16 * f + 8 * h + e;
-        float val = 0.f;
-        if (c < 2) {
-            const float2 w = a.wave[row * a.n + y];
-            val = (c == 0 ? w.x : w.y) * a.inv_psi0;
-        } else if (c < 2 + a.ns) {
-            val = a.sigma * a.slices[(b * a.ns + (c - 2)) * npix + x * a.n + y];
-        }
-        v[e] = val;
+    for (int i = 0; i < 4; ++i) {
+        int y = 32 * (w + 4 * i) + l32;
+        y = y < n ? y : n - 1;
+        wv[i] = wrow[y];
+#pragma unroll
+        for (int f = 0; f < K1F; ++f)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int c = 16 * f + 8 * h + e;
+                const int pl = c - 2 < 0 ? 0 : (c - 2 > last ? last : c - 2);
+                raw[i][f][e] = srow[pl * npix + y];
+            }
     }
-    uint32_t q[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) q[j] = f32x2_to_bf16x2(v[2 * j], v[2 * j + 1]);
-    return __builtin_bit_cast(bf16x8_t, q);
+    for (int i = 0; i < 4; ++i) {
+        const bool ok = 32 * (w + 4 * i) < n;
+#pragma unroll
+        for (int f = 0; f < K1F; ++f) {
+            float v[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int c = 16 * f + 8 * h + e;
+                const float val = c == 0 ? wv[i].x * a.inv_psi0
+                                : c == 1 ? wv[i].y * a.inv_psi0
+                                : (c < 2 + a.ns ? a.sigma * raw[i][f][e] : 0.f);
+                v[e] = ok ? val : 0.f;
+            }
+            uint32_t q[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) q[j] = f32x2_to_bf16x2(v[2 * j], v[2 * j + 1]);
+            vf[i][f] = __builtin_bit_cast(bf16x8_t, q);
+        }
+    }
 }
 
 template <int K1F, int K2F, int MODE>
@@ -265,16 +295,16 @@ __global__ __launch_bounds__(256) void k_fno_rows(FnoRowArgs a) {
     const uint16_t* vrow = a.vin + row * n * k1;
 
     bf16x8_t vf[4][K1F], gf[4][K2F], ff[4][2];
+    if constexpr (MODE & kRowPack) fno_packed_frags<K1F>(a, row, w, l32, h, vf);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int y0 = 32 * (w + 4 * i), y = y0 + l32;
         const bool ok = y0 < n;
+        if constexpr (!(MODE & kRowPack)) {
 #pragma unroll
-        for (int f = 0; f < K1F; ++f) {
-            vf[i][f] = bf16x8_t{};
-            if (ok) {
-                if constexpr (MODE & kRowPack) vf[i][f] = fno_packed_frag(a, row, y, f, h);
-                else vf[i][f] = *(const bf16x8_t*)(vrow + (int64_t)y * k1 + 16 * f + 8 * h);
+            for (int f = 0; f < K1F; ++f) {
+                vf[i][f] = bf16x8_t{};
+                if (ok) vf[i][f] = *(const bf16x8_t*)(vrow + (int64_t)y * k1 + 16 * f + 8 * h);
             }
         }
 #pragma unroll
