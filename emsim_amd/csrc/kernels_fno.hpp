@@ -296,21 +296,38 @@ __global__ __launch_bounds__(256) void k_fno_rows(FnoRowArgs a) {
 
     bf16x8_t vf[4][K1F], gf[4][K2F], ff[4][2];
     if constexpr (MODE & kRowPack) fno_packed_frags<K1F>(a, row, w, l32, h, vf);
+    // Without the next y-transform (TOUT) the loads are unconditional, from
+    // clamped indices and masked afterwards, so all of a wave's fragment
+    // loads are in flight together instead of each tile's waiting behind its
+    // branch; that measured faster for the last layer and slower for the
+    // middle ones, which keep the branches.
+    constexpr bool UNC = !TOUT;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const int y0 = 32 * (w + 4 * i), y = y0 + l32;
+        const int y0 = 32 * (w + 4 * i);
         const bool ok = y0 < n;
+        const int y = (UNC && !ok ? 0 : y0) + l32;
         if constexpr (!(MODE & kRowPack)) {
 #pragma unroll
             for (int f = 0; f < K1F; ++f) {
-                vf[i][f] = bf16x8_t{};
-                if (ok) vf[i][f] = *(const bf16x8_t*)(vrow + (int64_t)y * k1 + 16 * f + 8 * h);
+                if constexpr (UNC) {
+                    const bf16x8_t t = *(const bf16x8_t*)(vrow + (int64_t)y * k1 + 16 * f + 8 * h);
+                    vf[i][f] = ok ? t : bf16x8_t{};
+                } else {
+                    vf[i][f] = bf16x8_t{};
+                    if (ok) vf[i][f] = *(const bf16x8_t*)(vrow + (int64_t)y * k1 + 16 * f + 8 * h);
+                }
             }
         }
 #pragma unroll
         for (int f = 0; f < K2F; ++f) {
             gf[i][f] = bf16x8_t{};
-            if (HAS_U && ok) gf[i][f] = *(const bf16x8_t*)(a.gy + (int64_t)y * m2x2 + 16 * f + 8 * h);
+            if constexpr (HAS_U && UNC) {
+                const bf16x8_t t = *(const bf16x8_t*)(a.gy + (int64_t)y * m2x2 + 16 * f + 8 * h);
+                gf[i][f] = ok ? t : bf16x8_t{};
+            } else if constexpr (HAS_U) {
+                if (ok) gf[i][f] = *(const bf16x8_t*)(a.gy + (int64_t)y * m2x2 + 16 * f + 8 * h);
+            }
         }
 #pragma unroll
         for (int f = 0; f < 2; ++f) {
