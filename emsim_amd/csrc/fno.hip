@@ -90,7 +90,7 @@ namespace {
 struct FnoWs {
     uint16_t* v[9];      // activations (bf16, P x C): v[0] lifting .. v[L]
     uint16_t* z[8];      // pre-activations of each layer (training)
-    float* ty;           // (B n, 2 M2 C)
+    float* ty;           // (B n, 2 M2 C): float32, or bf16 written by the fused rows
     float* pq[8];        // (B, 2 M1, 2 M2 C) per layer (training keeps all)
     float* ob;           // (B, 2 M1, M2 C)
     float* u;            // (B, 2 n, M2 C); bf16 in the fused forward
@@ -352,25 +352,46 @@ namespace {
 
 // The 32-channel network with at most 16 y-modes on rows of at most 512
 // pixels runs its row-local work in k_fno_rows; anything else in k_gemm.
+// columns per lane load of k_fno_xfwd (1, 2 or 4)
+#ifndef EMSIM_XFWD_V
+#define EMSIM_XFWD_V 2
+#endif
+constexpr int kXfwdV = EMSIM_XFWD_V;
+
 bool fused_rows_ok(const emsim_hip_fno* f) {
     return f->C == 32 && f->n <= 512 && f->n % 32 == 0 && (2 * f->M2 == 16 || 2 * f->M2 == 32) &&
            f->cin_pad <= 32;
 }
 
-// x-transform of T, per-mode mix, inverse x-transform -> U (one layer)
-// x-transform, channel mix and inverse x-transform of layer l.  u16: U is
-// written as bf16, what the fused row kernel reads (it would round float32
-// U to bf16 the same way when staging it).
+// x-transform, channel mix and inverse x-transform of layer l (T -> U).
+// u16: the fused row kernels' bf16 T and U (they are rounded to bf16 where
+// they are read in any case); then the x-transforms run in k_fno_xfwd /
+// k_fno_xinv where the shapes allow, else in k_gemm.
 int spectral_middle(const emsim_hip_fno* f, const FnoWs& w, int batch, int l, float* pq,
                     hipStream_t s, bool u16 = false) {
     const int n = f->n, C = f->C, M1 = f->M1, M2 = f->M2;
     int rc;
-    emsim_gemm_desc d = gemm_desc();
-    d.m = 2 * M1; d.n = 2 * M2 * C; d.k = n; d.batch = batch;
-    d.a = f->fx2; d.a_type = EMSIM_DT_BF16; d.a_sm = n; d.a_sk = 1;
-    d.b = w.ty; d.b_type = EMSIM_DT_F32; d.b_bs = (int64_t)n * 2 * M2 * C; d.b_sk = 2 * M2 * C; d.b_sn = 1;
-    d.c = pq; d.c_type = EMSIM_DT_F32; d.c_bs = (int64_t)2 * M1 * 2 * M2 * C; d.c_sm = 2 * M2 * C; d.c_sn = 1;
-    if ((rc = launch_gemm(d, s))) return rc;
+    // the fused path's own x-transform kernels where their shapes allow
+    const bool xk = u16 && 2 * M1 <= 64 && 2 * M1 % 16 == 0 && n % 64 == 0 &&
+                    (M2 * C == 256 || M2 * C == 512) && (int64_t)batch * 2 * n / 32 <= 0x7fffffff;
+    if (xk) {
+        const int nc = 2 * M2 * C;
+        const dim3 grid((unsigned)((int64_t)batch * nc / (32 * kXfwdV)));
+        if (2 * M1 <= 32)
+            EMSIM_LAUNCH(0, (k_fno_xfwd<1, kXfwdV>), grid, dim3(256), 0, s, f->fx2,
+                         (const uint16_t*)w.ty, pq, n, 2 * M1, nc);
+        else
+            EMSIM_LAUNCH(0, (k_fno_xfwd<2, kXfwdV>), grid, dim3(256), 0, s, f->fx2,
+                         (const uint16_t*)w.ty, pq, n, 2 * M1, nc);
+    } else {
+        emsim_gemm_desc d = gemm_desc();
+        d.m = 2 * M1; d.n = 2 * M2 * C; d.k = n; d.batch = batch;
+        d.a = f->fx2; d.a_type = EMSIM_DT_BF16; d.a_sm = n; d.a_sk = 1;
+        d.b = w.ty; d.b_type = u16 ? EMSIM_DT_BF16 : EMSIM_DT_F32;
+        d.b_bs = (int64_t)n * 2 * M2 * C; d.b_sk = 2 * M2 * C; d.b_sn = 1;
+        d.c = pq; d.c_type = EMSIM_DT_F32; d.c_bs = (int64_t)2 * M1 * 2 * M2 * C; d.c_sm = 2 * M2 * C; d.c_sn = 1;
+        if ((rc = launch_gemm(d, s))) return rc;
+    }
     const float2* R = (const float2*)(f->params + f->off_r[l]);
     if (C == 32)
         EMSIM_LAUNCH(0, k_fno_mix<32>, dim3(M1 * M2), dim3(256), 0, s, (const float*)pq, R, w.ob, batch, M1, M2);
@@ -378,7 +399,17 @@ int spectral_middle(const emsim_hip_fno* f, const FnoWs& w, int batch, int l, fl
         EMSIM_LAUNCH(0, k_fno_mix<16>, dim3(M1 * M2), dim3(256), 0, s, (const float*)pq, R, w.ob, batch, M1, M2);
     else
         EMSIM_LAUNCH(0, k_fno_mix<64>, dim3(M1 * M2), dim3(256), 0, s, (const float*)pq, R, w.ob, batch, M1, M2);
-    d = gemm_desc();
+    if (xk) {
+        const dim3 grid((unsigned)((int64_t)batch * 2 * n / 32));
+        if (M2 * C == 512)
+            EMSIM_LAUNCH(0, k_fno_xinv<512>, grid, dim3(256), 0, s, f->gx, (const float*)w.ob,
+                         (uint16_t*)w.u, 2 * n, 2 * M1);
+        else
+            EMSIM_LAUNCH(0, k_fno_xinv<256>, grid, dim3(256), 0, s, f->gx, (const float*)w.ob,
+                         (uint16_t*)w.u, 2 * n, 2 * M1);
+        return EMSIM_HIP_OK;
+    }
+    emsim_gemm_desc d = gemm_desc();
     d.m = 2 * n; d.n = M2 * C; d.k = 2 * M1; d.batch = batch;
     d.a = f->gx; d.a_type = EMSIM_DT_BF16; d.a_sm = 2 * M1; d.a_sk = 1;
     d.b = w.ob; d.b_type = EMSIM_DT_F32; d.b_bs = (int64_t)2 * M1 * M2 * C; d.b_sk = M2 * C; d.b_sn = 1;
@@ -451,7 +482,7 @@ int forward_fused(const emsim_hip_fno* f, const FnoWs& w, const void* x, float* 
     a.n = f->n;
     a.gy = f->gy;
     a.m2x2 = 2 * f->M2;
-    a.T = w.ty;
+    a.T = (uint16_t*)w.ty;
     // lifting, and the y-transform of its output
     a.vin = (const uint16_t*)x;
     a.k1 = f->cin_pad;

@@ -191,7 +191,7 @@ struct FnoRowArgs {
     uint16_t* vout;        // (rows, n, 32) bf16, or null
     uint16_t* zout;        // (rows, n, 32) bf16 pre-activation, or null
     const uint16_t* fy;    // (2 M2, n) bf16, or null (no y-transform of the output)
-    float* T;              // (rows, 2 M2, 32) float32
+    uint16_t* T;           // (rows, 2 M2, 32) bf16
     const uint16_t* q16;   // (32, 2) bf16 projection, or null
     const float* qb;       // (2)
     float* out;            // (rows, n, 2) float32
@@ -426,13 +426,17 @@ __global__ __launch_bounds__(256) void k_fno_rows(FnoRowArgs a) {
         red[(w * 32 + c) * 33 + l32] = acc[v];
     }
     __syncthreads();
-    float* trow = a.T + row * m2x2 * 32;
-    for (int i = threadIdx.x; i < m2x2 * 32; i += 256) {
-        const int m = i >> 5, c = i & 31;
-        float s = 0.f;
+    // T leaves as bf16: its one consumer, the x-transform, rounds it to bf16
+    uint32_t* trow = (uint32_t*)(a.T + row * m2x2 * 32);
+    for (int i = threadIdx.x; i < m2x2 * 16; i += 256) {
+        const int m = i >> 4, c = 2 * (i & 15);
+        float s0 = 0.f, s1 = 0.f;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) s += red[(q * 32 + c) * 33 + m];
-        trow[i] = s;
+        for (int q = 0; q < 4; ++q) {
+            s0 += red[(q * 32 + c) * 33 + m];
+            s1 += red[(q * 32 + c + 1) * 33 + m];
+        }
+        trow[i] = f32x2_to_bf16x2(s0, s1);
     }
 }
 
@@ -485,6 +489,191 @@ __global__ __launch_bounds__(256) void k_fno_mix(const float* __restrict__ PQ,
             ob[(int64_t)k1 * ob_ld + k2 * C + co] = orr;
             ob[(int64_t)(M1 + k1) * ob_ld + k2 * C + co] = oi;
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Forward x-transform of a spectral layer on the fused path:
+//   PQ[b] (2 M1, nc) = Fx2 (2 M1, n) T[b] (n, nc),   nc = 2 M2 C,
+// with 2 M1 <= 32 MT and T in bf16.  One block per (image b, 32 V
+// columns); wave w sums x in [w n/4, (w+1) n/4) into MT x V tiles of
+// 32 x 32, and the four partial sums are added in LDS in two rounds.  Each
+// lane loads V adjacent columns of 8 consecutive x straight from T: column
+// j of the wave's V tiles is column V l32 + j of the block's slab, so the V
+// values of one load are B fragments of V tiles, and a wave's load covers
+// whole runs of two rows.  T is read once, with no LDS staging; Fx2
+// (2 M1 n bf16) stays in cache.  Needs n % 64 == 0 and nc % (32 V) == 0.
+// Same operands as k_gemm on float32 T (which it rounds to bf16); only the
+// float32 summation order differs.
+// ---------------------------------------------------------------------------
+// V adjacent bf16 of one row, as one load
+template <int V> struct Bf16V;
+template <> struct Bf16V<1> { typedef uint16_t t; };
+template <> struct Bf16V<2> { typedef uint32_t t; };
+template <> struct Bf16V<4> { typedef uint2 t; };
+
+// bits of element j of a Bf16V load, in the low half
+template <int V>
+__device__ __forceinline__ uint32_t bv_get(const typename Bf16V<V>::t& x, int j) {
+    if constexpr (V == 1) return x;
+    else if constexpr (V == 2) return j == 0 ? (x & 0xffffu) : (x >> 16);
+    else {
+        const uint32_t wd = j < 2 ? x.x : x.y;
+        return (j & 1) ? (wd >> 16) : (wd & 0xffffu);
+    }
+}
+
+template <int MT, int V>
+__global__ __launch_bounds__(256) void k_fno_xfwd(const uint16_t* __restrict__ fx2,
+                                                  const uint16_t* __restrict__ T,
+                                                  float* __restrict__ PQ, int n, int m2x1,
+                                                  int nc) {
+    typedef typename Bf16V<V>::t fv_t;
+    constexpr int TS = 32 * 33;   // one padded 32 x 32 tile in LDS
+    __shared__ float red[2][MT * V * TS];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, l32 = lane & 31, h = lane >> 5;
+    const int slabs = nc / (32 * V);
+    const int64_t b = blockIdx.x / slabs;
+    const int c0 = (blockIdx.x % slabs) * 32 * V;
+    const int kq = n >> 2, xb = w * kq;
+    bool arow[MT];
+    const uint16_t* ap[MT];
+#pragma unroll
+    for (int i = 0; i < MT; ++i) {
+        arow[i] = 32 * i + l32 < m2x1;
+        ap[i] = fx2 + (int64_t)(arow[i] ? 32 * i + l32 : 0) * n + 8 * h;
+    }
+    const uint16_t* tp = T + b * (int64_t)n * nc + c0 + V * l32 + (int64_t)(8 * h) * nc;
+    f32x16_t acc[MT][V];
+#pragma unroll
+    for (int i = 0; i < MT; ++i)
+#pragma unroll
+        for (int j = 0; j < V; ++j) acc[i][j] = f32x16_t{};
+    // two k-steps of 16 per iteration: every load of both is issued before
+    // the first MFMA
+    for (int x0 = xb; x0 < xb + kq; x0 += 32) {
+        const bool two = x0 + 16 < xb + kq;   // wave-uniform
+        bf16x8_t fa[2][MT];
+        fv_t tv[2][8];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int xs = s == 1 && !two ? x0 : x0 + 16 * s;
+#pragma unroll
+            for (int i = 0; i < MT; ++i) fa[s][i] = *(const bf16x8_t*)(ap[i] + xs);
+#pragma unroll
+            for (int r = 0; r < 8; ++r) tv[s][r] = *(const fv_t*)(tp + (int64_t)(xs + r) * nc);
+        }
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            if (s == 1 && !two) break;
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                uint32_t q[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    q[r] = bv_get<V>(tv[s][2 * r], j) | (bv_get<V>(tv[s][2 * r + 1], j) << 16);
+                const bf16x8_t fb = __builtin_bit_cast(bf16x8_t, q);
+#pragma unroll
+                for (int i = 0; i < MT; ++i) {
+                    const bf16x8_t a = arow[i] ? fa[s][i] : bf16x8_t{};
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, fb, acc[i][j], 0, 0, 0);
+                }
+            }
+        }
+    }
+    // partial sums: waves 2, 3 -> LDS, added by waves 0, 1; wave 1 -> LDS,
+    // added by wave 0, which leaves the block's total in red[0]
+    auto put = [&](float* dst) {
+#pragma unroll
+        for (int i = 0; i < MT; ++i)
+#pragma unroll
+            for (int j = 0; j < V; ++j)
+#pragma unroll
+                for (int v = 0; v < 16; ++v)
+                    dst[(i * V + j) * TS + (8 * (v >> 2) + 4 * h + (v & 3)) * 33 + l32] = acc[i][j][v];
+    };
+    auto add = [&](const float* src) {
+#pragma unroll
+        for (int i = 0; i < MT; ++i)
+#pragma unroll
+            for (int j = 0; j < V; ++j)
+#pragma unroll
+                for (int v = 0; v < 16; ++v)
+                    acc[i][j][v] += src[(i * V + j) * TS + (8 * (v >> 2) + 4 * h + (v & 3)) * 33 + l32];
+    };
+    if (w >= 2) put(red[w - 2]);
+    __syncthreads();
+    if (w < 2) add(red[w]);
+    __syncthreads();
+    if (w == 1) put(red[0]);
+    __syncthreads();
+    if (w == 0) {
+        add(red[0]);
+        put(red[0]);
+    }
+    __syncthreads();
+    float* out = PQ + b * (int64_t)m2x1 * nc + c0;
+    for (int e = threadIdx.x; e < m2x1 * 32 * V; e += 256) {
+        const int m = e / (32 * V), c = e % (32 * V), j = c % V, cl = c / V;
+        out[(int64_t)m * nc + c] = red[0][((m >> 5) * V + j) * TS + (m & 31) * 33 + cl];
+    }
+}
+
+// Inverse x-transform of a spectral layer on the fused path, written as the
+// bf16 Uhat the fused row kernel reads:
+//   U[b] (2 n, MC) = Gx (2 n, 2 M1) Ob[b] (2 M1, MC),   MC = M2 C.
+// One block per (image b, 32 rows of U); wave w computes the 32 x 32 column
+// tiles w, w + 4, ... (K = 2 M1, a multiple of 16 up to 64).  The block's
+// rows are contiguous in U, so they are staged in LDS and written with
+// 16-byte stores.  Rounding as k_gemm: Ob and the result to bf16, nearest
+// even, with the same k order.
+template <int MC>
+__global__ __launch_bounds__(256) void k_fno_xinv(const uint16_t* __restrict__ gx,
+                                                  const float* __restrict__ Ob,
+                                                  uint16_t* __restrict__ U, int n2, int k) {
+    constexpr int LD = MC + 8, TPW = MC / 128, KS = 4;
+    __shared__ __align__(16) uint16_t tile[32 * LD];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, l32 = lane & 31, h = lane >> 5;
+    const int rtiles = n2 >> 5;
+    const int64_t b = blockIdx.x / rtiles;
+    const int r0 = (blockIdx.x % rtiles) * 32;
+    const float* ob = Ob + b * (int64_t)k * MC + (int64_t)(8 * h) * MC;
+    const int steps = k >> 4;   // 1 .. 4, wave-uniform
+    bf16x8_t fa[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        const int sc = s < steps ? s : 0;
+        fa[s] = *(const bf16x8_t*)(gx + (int64_t)(r0 + l32) * k + 16 * sc + 8 * h);
+    }
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+        const int col = (w + 4 * t) * 32 + l32;
+        float bv[KS][8];
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const int sc = s < steps ? s : 0;
+#pragma unroll
+            for (int r = 0; r < 8; ++r) bv[s][r] = ob[(int64_t)(16 * sc + r) * MC + col];
+        }
+        f32x16_t acc = {};
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            if (s >= steps) break;
+            uint32_t q[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) q[r] = f32x2_to_bf16x2(bv[s][2 * r], bv[s][2 * r + 1]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[s], __builtin_bit_cast(bf16x8_t, q),
+                                                          acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 16; ++v)
+            tile[(8 * (v >> 2) + 4 * h + (v & 3)) * LD + col] = f32_to_bf16(acc[v]);
+    }
+    __syncthreads();
+    uint16_t* ub = U + (b * n2 + r0) * (int64_t)MC;
+    for (int i = threadIdx.x; i < 32 * MC / 8; i += 256) {
+        const int row = i / (MC / 8), c8 = i % (MC / 8);
+        *(uint4*)(ub + (int64_t)row * MC + 8 * c8) = *(const uint4*)(tile + row * LD + 8 * c8);
     }
 }
 

@@ -206,6 +206,34 @@ def test_fused_rows_equal_generic_path(hip, net):
     assert rel < 1e-3
 
 
+def test_full_width_modes_match_oracle_and_generic(hip):
+    """16 modes (2 M1 = 64 x-transform rows, M2 C = 512 columns: the shapes of
+    the 512^2 network) through the fused path's x-transform kernels, against
+    the oracle and against the all-GEMM sequence."""
+    from emsim_amd.learned import FNO
+    m, L = 16, 2
+    f = FNO(NN, CIN, CC, m, L)
+    p = fo.random_params(CIN, CC, m, L, seed=21)
+    f.set_params(p)
+    rng = np.random.default_rng(22)
+    x = np.zeros((BB, NN, NN, f.cin_pad))
+    x[..., :CIN] = rng.normal(size=(BB, NN, NN, CIN))
+    xt = _dev(x, torch.bfloat16)
+    x_bf = xt.to(torch.float32).cpu().numpy().astype(np.float64)
+    got = f.forward(xt).cpu().numpy()
+    emu, _ = fo.forward(f.get_params(), x_bf, m, emulate_bf16=True)
+    rel_emu = np.linalg.norm(got - emu) / np.linalg.norm(emu)
+    hip.call("emsim_hip_fno_force_generic", 1)
+    try:
+        gen = f.forward(xt).cpu().numpy()
+    finally:
+        hip.call("emsim_hip_fno_force_generic", 0)
+    rel_gen = np.linalg.norm(got - gen) / np.linalg.norm(gen)
+    print(f"16 modes: vs bf16-emulating oracle {rel_emu:.2e}, vs generic {rel_gen:.2e}")
+    assert rel_emu < 1e-3
+    assert rel_gen < 1e-3
+
+
 def test_infer_equals_pack_then_forward(net):
     """Packing fused into the lifting gives the same output bit for bit."""
     f, _, _, _ = net
