@@ -270,7 +270,9 @@ void row_grid(int64_t n_items, int& rpw, dim3& grid) {
 
 int64_t col_tiles_per_block(int64_t n_tiles) {
     if (g_col_tiles_per_block > 0) return g_col_tiles_per_block;
-    const int64_t tpb = n_tiles / 512;
+    // the fewest tiles per block that keep the grid within 512 blocks (two
+    // resident per CU) so that no second round of blocks trails the first
+    const int64_t tpb = ceil_div(n_tiles, 512);
     return tpb < 1 ? 1 : (tpb > 8 ? 8 : tpb);
 }
 
@@ -316,24 +318,48 @@ int launch_row_ifft_intensity(const float2* in, float* out, int64_t rows, const 
     return EMSIM_HIP_OK;
 }
 
+// Wave columns [lo, hi] that the multiplier zeroes whole (none by default):
+// pass B loads none of them, and with skip_tiles leaves out the column
+// tiles that hold only such columns (their contents are then left as they
+// are, for callers that never read them).
+struct ColDead {
+    int lo = 1 << 30, hi = -1;
+    bool skip_tiles = false;
+};
+
 template <int N, class Mul>
 int launch_col_mul_impl(float2* data, int ld, int ncols, int64_t img_stride, int n_img, Mul mul,
-                        const float2* tw, hipStream_t s) {
+                        const float2* tw, hipStream_t s, ColDead cd = ColDead{}) {
     using Cfg = ColCfg<N>;
     const int tiles_per_img = (int)ceil_div(ncols, Cfg::C);
-    const int64_t n_tiles = (int64_t)tiles_per_img * n_img;
+    DeadCols dc;
+    dc.lo = cd.lo;
+    dc.hi = cd.hi;
+    dc.live_per_img = tiles_per_img;
+    dc.skip_lo = tiles_per_img;
+    dc.skip_n = 0;
+    if (cd.skip_tiles && cd.lo <= cd.hi) {
+        // tiles [first, end) lie inside [lo, hi]
+        const int first = (int)ceil_div(cd.lo, Cfg::C), end = (cd.hi + 1) / Cfg::C;
+        if (end > first) {
+            dc.skip_lo = first;
+            dc.skip_n = end - first;
+            dc.live_per_img = tiles_per_img - dc.skip_n;
+        }
+    }
+    const int64_t n_tiles = (int64_t)dc.live_per_img * n_img;
     const int64_t tpb = col_tiles_per_block(n_tiles);
     // blocks pair up on an XCD (see k_col_fft_mul_ifft): a multiple of 16
     const dim3 grid((unsigned)(ceil_div(ceil_div(n_tiles, tpb), 16) * 16));
     EMSIM_LAUNCH(N, (k_col_fft_mul_ifft<N, Mul>), grid, dim3(Cfg::BLOCK), 0, s, data, ld, ncols,
-                 tiles_per_img, img_stride, n_tiles, (int)tpb, mul, tw);
+                 dc, img_stride, n_tiles, (int)tpb, mul, tw);
     return EMSIM_HIP_OK;
 }
 
 template <int N>
 int launch_col_prop(float2* data, int ld, int ncols, int64_t img_stride, int n_img,
-                    PropagatorMul mul, const float2* tw, hipStream_t s) {
-    return launch_col_mul_impl<N>(data, ld, ncols, img_stride, n_img, mul, tw, s);
+                    PropagatorMul mul, const float2* tw, hipStream_t s, ColDead cd) {
+    return launch_col_mul_impl<N>(data, ld, ncols, img_stride, n_img, mul, tw, s, cd);
 }
 
 template <int N>
@@ -399,8 +425,8 @@ int row_ifft_intensity(int n, const float2* in, float* out, int64_t rows, const 
     EMSIM_DISPATCH(n, launch_row_ifft_intensity, in, out, rows, tw, s);
 }
 int col_prop(int n, float2* data, int ld, int ncols, int64_t img_stride, int n_img,
-             PropagatorMul mul, const float2* tw, hipStream_t s) {
-    EMSIM_DISPATCH(n, launch_col_prop, data, ld, ncols, img_stride, n_img, mul, tw, s);
+             PropagatorMul mul, const float2* tw, hipStream_t s, ColDead cd = ColDead{}) {
+    EMSIM_DISPATCH(n, launch_col_prop, data, ld, ncols, img_stride, n_img, mul, tw, s, cd);
 }
 int col_lens(int n, float2* data, int ld, int ncols, int64_t img_stride, int n_img, LensMul mul,
              const float2* tw, hipStream_t s) {
@@ -436,6 +462,29 @@ PropagatorMul make_propagator(const emsim_hip_wave_propagator* w, float dz) {
     p.lpd = lp * dz;
     p.inv_npix = 1.0f / (float)(w->n1 * w->n2);
     return p;
+}
+
+// The wave columns the propagator zeroes whole.  Its mask keeps
+// f^2 = fx^2 + fy^2 <= filter^2 (PropagatorMul::apply), and in float32
+// round-to-nearest f^2 >= fy^2, so every column with fy^2 > filter^2 is
+// zero; fy^2 is formed by the same float32 operations as on the device.
+// Reported only as one contiguous run of column indices (else none).
+ColDead dead_columns(const PropagatorMul& p) {
+    ColDead d;
+    int lo = -1, hi = -1;
+    for (int j = 0; j < p.n2; ++j) {
+        const float fy = (float)freq_int(j, p.n2) * p.dfy;
+        const float fy2 = fy * fy;
+        if (fy2 <= p.filter2) continue;
+        if (lo < 0) lo = j;
+        else if (hi != j - 1) return ColDead{};
+        hi = j;
+    }
+    if (lo >= 0) {
+        d.lo = lo;
+        d.hi = hi;
+    }
+    return d;
 }
 
 LensMul make_lens(const emsim_hip_wave_propagator* w, float cs_mm, float defocus, float aperture) {
@@ -1035,7 +1084,7 @@ int launch_spec_sparse(const uint32_t* row_start, const uint16_t* bucket, const 
 template <int N>
 int launch_slice_pass_a(bool first, const float2* Y, float2* wave, int64_t n_pairs, int ldh,
                         float psi0, float factor, float* vout, int slice, int n_slices, int n1,
-                        const float2* tw, hipStream_t s) {
+                        ColDead cd, const float2* tw, hipStream_t s) {
     int rpw;
     dim3 grid;
     row_grid<N>(n_pairs, rpw, grid);
@@ -1044,20 +1093,20 @@ int launch_slice_pass_a(bool first, const float2* Y, float2* wave, int64_t n_pai
     if (first)
         EMSIM_LAUNCH(N, (k_slice_pass_a<N, true>), grid, block, 0, s,
                      (OpSlicePassA<N, true>{Y, wave, ldh, psi0, factor, vout, n_slices * plane,
-                                            slice * plane, n1}),
+                                            slice * plane, n1, cd.lo, cd.hi}),
                      n_pairs, rpw, tw);
     else
         EMSIM_LAUNCH(N, (k_slice_pass_a<N, false>), grid, block, 0, s,
                      (OpSlicePassA<N, false>{Y, wave, ldh, psi0, factor, vout, n_slices * plane,
-                                             slice * plane, n1}),
+                                             slice * plane, n1, cd.lo, cd.hi}),
                      n_pairs, rpw, tw);
     return EMSIM_HIP_OK;
 }
 
 template <int N>
 int launch_col_lensprop(float2* data, int ld, int ncols, int64_t img_stride, int n_img,
-                        LensPropMul mul, const float2* tw, hipStream_t s) {
-    return launch_col_mul_impl<N>(data, ld, ncols, img_stride, n_img, mul, tw, s);
+                        LensPropMul mul, const float2* tw, hipStream_t s, ColDead cd) {
+    return launch_col_mul_impl<N>(data, ld, ncols, img_stride, n_img, mul, tw, s, cd);
 }
 
 int spec_sparse(int n, const uint32_t* row_start, const uint16_t* bucket, const int2* twq,
@@ -1069,13 +1118,13 @@ int spec_sparse(int n, const uint32_t* row_start, const uint16_t* bucket, const 
 }
 int slice_pass_a(int n, bool first, const float2* Y, float2* wave, int64_t n_pairs, int ldh,
                  float psi0, float factor, float* vout, int slice, int n_slices, int n1,
-                 const float2* tw, hipStream_t s) {
+                 ColDead cd, const float2* tw, hipStream_t s) {
     EMSIM_DISPATCH(n, launch_slice_pass_a, first, Y, wave, n_pairs, ldh, psi0, factor, vout,
-                   slice, n_slices, n1, tw, s);
+                   slice, n_slices, n1, cd, tw, s);
 }
 int col_lensprop(int n, float2* data, int ld, int ncols, int64_t img_stride, int n_img,
-                 LensPropMul mul, const float2* tw, hipStream_t s) {
-    EMSIM_DISPATCH(n, launch_col_lensprop, data, ld, ncols, img_stride, n_img, mul, tw, s);
+                 LensPropMul mul, const float2* tw, hipStream_t s, ColDead cd) {
+    EMSIM_DISPATCH(n, launch_col_lensprop, data, ld, ncols, img_stride, n_img, mul, tw, s, cd);
 }
 
 struct FusedWorkspace {
@@ -1272,21 +1321,36 @@ int simulate_core(const emsim_hip_slice_builder* b, const emsim_hip_wave_propaga
     const int64_t npix = (int64_t)b->n1 * b->n2;
     const int64_t rows = (int64_t)batch * b->n1;
     const PropagatorMul prop = make_propagator(w, (float)b->dz);
+    // Columns the propagator zeroes whole are left out between the passes:
+    // the wave buffer starts as zeros, pass A never stores into them and
+    // pass B skips the tiles that hold only such columns (zeros from the
+    // others).  The last pass B runs every tile.
+    ColDead dead = dead_columns(prop);
+    dead.skip_tiles = true;
+    ColDead dead_last = dead;
+    dead_last.skip_tiles = false;
+    if (dead.lo <= dead.hi) {
+        const int64_t n_wave = (int64_t)batch * npix;
+        EMSIM_LAUNCH(0, k_fill_complex, dim3(grid_for(n_wave, 256)), dim3(256), 0, s, wave,
+                     n_wave, 0.f);
+    }
     for (int sl = 0; sl < b->n_slices; ++sl) {
+        const bool last = sl == b->n_slices - 1;
         rc = spec_sparse(b->n1, ws.row_start, ws.bucket, twq, b->sf, ws.Y, sl, b->n_slices,
                          b->n_elems, b->H, ldh, b->n2, symmetric_bandlimit ? 1 : 0, r * r,
                          inv_npix, batch, b->tw, s);
         if (rc) return rc;
         rc = slice_pass_a(b->n2, sl == 0, ws.Y, wave, rows / 2, ldh, psi0, w->factor, slices_out,
-                          sl, b->n_slices, b->n1, b->tw, s);
+                          sl, b->n_slices, b->n1, dead, b->tw, s);
         if (rc) return rc;
-        if (mode == 0 && sl == b->n_slices - 1) {
+        if (mode == 0 && last) {
             LensPropMul lp;
             lp.p = prop;
             lp.l = make_lens(w, cs_mm, defocus, aperture);
-            rc = col_lensprop(b->n1, wave, b->n2, b->n2, npix, batch, lp, b->tw, s);
+            rc = col_lensprop(b->n1, wave, b->n2, b->n2, npix, batch, lp, b->tw, s, dead_last);
         } else {
-            rc = col_prop(b->n1, wave, b->n2, b->n2, npix, batch, prop, b->tw, s);
+            rc = col_prop(b->n1, wave, b->n2, b->n2, npix, batch, prop, b->tw, s,
+                          last ? dead_last : dead);
         }
         if (rc) return rc;
     }

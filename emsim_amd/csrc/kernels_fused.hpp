@@ -587,6 +587,12 @@ struct OpSlicePassA {
     int64_t v_img_stride;  // nS n1 N
     int64_t v_slice_off;   // slice n1 N
     int rows_per_img;      // n1
+    // wave columns [dead_lo, dead_hi] are zero after every pass B (the
+    // propagator's mask); the buffer holds zeros there from the start of
+    // the run, and this pass never stores into them, so they are loaded as
+    // they are
+    int dead_lo, dead_hi;
+    __device__ __forceinline__ bool dead(int k) const { return k >= dead_lo && k <= dead_hi; }
     // raw half-spectrum values of the pair's two rows at this thread's k
     __device__ __forceinline__ void load_y(int64_t pair, int t, bool ok, float2 (&ya)[EPT],
                                            float2 (&yb)[EPT]) const {
@@ -653,7 +659,8 @@ struct OpSlicePassA {
         if (!ok) return;
         float2* dst = wave + (pair * 2 + row) * (int64_t)N;
 #pragma unroll
-        for (int m = 0; m < EPT; ++m) dst[t + m * T] = w[m];
+        for (int m = 0; m < EPT; ++m)
+            if (!dead(t + m * T)) dst[t + m * T] = w[m];
     }
 };
 

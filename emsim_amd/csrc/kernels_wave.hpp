@@ -25,7 +25,7 @@ namespace emsim {
 // Signed frequency of FFT bin i: the reference's ifftshift map
 // is = i + (i < (n+1)/2 ? n/2 : -(n+1)/2), then is - floor(n/2)
 // (WavePropagator.cpp:66-70).
-__device__ __forceinline__ int freq_int(int i, int n) {
+__host__ __device__ __forceinline__ int freq_int(int i, int n) {
     const int is = i + (i < (n + 1) / 2 ? n / 2 : -((n + 1) / 2));
     return is - n / 2;
 }
@@ -327,9 +327,19 @@ __device__ __forceinline__ int64_t col_tile(unsigned L, int tpb, int r) {
     const int64_t pair = (int64_t)(k / 2) * kColXcds + x;
     return (pair * tpb + r) * 2 + (k % 2);
 }
+// Wave columns j in [lo, hi] that the propagator zeroes whole (none: lo >
+// hi).  Column tiles skip_lo .. skip_lo + skip_n - 1 of each image hold
+// only such columns; the launch runs the other live_per_img tiles.
+struct DeadCols {
+    int lo, hi;
+    int live_per_img, skip_lo, skip_n;
+    __device__ __forceinline__ bool dead(int j) const { return j >= lo && j <= hi; }
+    __device__ __forceinline__ int tile(int tl) const { return tl < skip_lo ? tl : tl + skip_n; }
+};
+
 template <int N, class Mul>
 __global__ __launch_bounds__(ColCfg<N>::BLOCK, EMSIM_COL_MIN_WAVES) void k_col_fft_mul_ifft(
-    float2* data, int ld, int ncols, int tiles_per_img, int64_t img_stride, int64_t n_tiles,
+    float2* data, int ld, int ncols, DeadCols dc, int64_t img_stride, int64_t n_tiles,
     int tpb, Mul mul, const float2* __restrict__ tw) {
     using Cfg = ColCfg<N>;
     constexpr int EPT = Cfg::EPT;
@@ -343,9 +353,11 @@ __global__ __launch_bounds__(ColCfg<N>::BLOCK, EMSIM_COL_MIN_WAVES) void k_col_f
     mul.prepare(lds_px, threadIdx.x, Cfg::BLOCK);
     if constexpr (Mul::kTable) __syncthreads();
 
+    // ok: the column exists (stored); a dead one is not loaded (its
+    // product with the propagator is zero)
     auto locate = [&](int64_t g, int& col, bool& ok) -> float2* {
-        const int64_t img = g / tiles_per_img;
-        col = (int)(g - img * tiles_per_img) * Cfg::C + c;
+        const int64_t img = g / dc.live_per_img;
+        col = dc.tile((int)(g - img * dc.live_per_img)) * Cfg::C + c;
         ok = g < n_tiles && col < ncols;
         return data + img * img_stride + col;
     };
@@ -355,9 +367,12 @@ __global__ __launch_bounds__(ColCfg<N>::BLOCK, EMSIM_COL_MIN_WAVES) void k_col_f
     int col;
     bool ok;
     float2* base = locate(col_tile(blockIdx.x, tpb, 0), col, ok);
+    {
+        const bool live = ok && !dc.dead(col);
 #pragma unroll
-    for (int m = 0; m < EPT; ++m)
-        cur[m] = ok ? base[(int64_t)(t + m * Cfg::T) * ld] : make_float2(0.f, 0.f);
+        for (int m = 0; m < EPT; ++m)
+            cur[m] = live ? base[(int64_t)(t + m * Cfg::T) * ld] : make_float2(0.f, 0.f);
+    }
     twc.fence();
     for (int r = 0; r < tpb; ++r) {
         int col_n = 0;
@@ -365,9 +380,10 @@ __global__ __launch_bounds__(ColCfg<N>::BLOCK, EMSIM_COL_MIN_WAVES) void k_col_f
         float2* base_n = nullptr;
         if (r + 1 < tpb) {
             base_n = locate(col_tile(blockIdx.x, tpb, r + 1), col_n, ok_n);
+            const bool live_n = ok_n && !dc.dead(col_n);
 #pragma unroll
             for (int m = 0; m < EPT; ++m)
-                nxt[m] = ok_n ? base_n[(int64_t)(t + m * Cfg::T) * ld] : make_float2(0.f, 0.f);
+                nxt[m] = live_n ? base_n[(int64_t)(t + m * Cfg::T) * ld] : make_float2(0.f, 0.f);
         }
         const float2 py = mul.column(col);
         fft_n<N, EPT, false>(cur, re, im, t, twc);
