@@ -28,6 +28,12 @@ if not LIB_PATH.exists():
 
 _lib = ctypes.CDLL(str(LIB_PATH))
 
+# the ABI the signatures below describe (EMSIM_HIP_ABI_VERSION in the header)
+ABI_VERSION = 2
+if _lib.emsim_hip_abi_version() != ABI_VERSION:
+    raise ImportError(f"{LIB_PATH} reports ABI {_lib.emsim_hip_abi_version()}, this binding "
+                      f"expects {ABI_VERSION}: rebuild it (python emsim_amd/csrc/build.py)")
+
 EINVAL, EUNSUPPORTED, EWORKSPACE, EDEVICE, ENOMEM = -1, -2, -3, -4, -5
 
 _SIGNATURES = {

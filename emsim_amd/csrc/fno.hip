@@ -538,7 +538,7 @@ int emsim_hip_fno_forward(emsim_hip_fno* f, const void* x_dev, float* out_dev, i
     hipStream_t s = (hipStream_t)stream;
     if (fused_rows_ok(f) && !g_fno_force_generic)
         return forward_fused(f, w, x_dev, out_dev, batch, training, s);
-    const int n = f->n, C = f->C, M1 = f->M1, M2 = f->M2;
+    const int n = f->n, C = f->C, M2 = f->M2;
     const int64_t P = (int64_t)batch * n * n;
     int rc;
     // lifting

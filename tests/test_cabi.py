@@ -26,6 +26,15 @@ def test_header_declares_the_expected_entry_points():
         assert must in names
 
 
+def test_abi_version_agrees_across_header_binding_and_library():
+    """build() and the binding's import check both compare against
+    hip_ext.ABI_VERSION; it must be the header's EMSIM_HIP_ABI_VERSION."""
+    m = re.search(r"#define\s+EMSIM_HIP_ABI_VERSION\s+(\d+)", HEADER.read_text())
+    assert m
+    hip_ext = pytest.importorskip("emsim_amd.backend.hip_ext")
+    assert hip_ext.ABI_VERSION == int(m.group(1)) == hip_ext.abi_version()
+
+
 def test_library_exports_every_declared_symbol():
     hip_ext = pytest.importorskip("emsim_amd.backend.hip_ext")
     lib = ctypes.CDLL(str(hip_ext.LIB_PATH))
