@@ -292,6 +292,27 @@ __device__ __forceinline__ void spec_fft(float2 (&v)[EPT], float* re, float* im,
     else fft_n<N, EPT, INV>(v, re, im, t, tw);
 }
 
+// Copy src[0, count) to LDS: in each round a thread issues all PER of its
+// loads before its stores, from indices clamped into range, so no lane's
+// load waits behind a branch or behind the previous entry's store.
+template <int PER, typename T>
+__device__ __forceinline__ void stage_lds(T* dst, const T* __restrict__ src, uint32_t count,
+                                          int tid, int nthreads) {
+    for (uint32_t base = 0; base < count; base += (uint32_t)(PER * nthreads)) {
+        T v[PER];
+#pragma unroll
+        for (int r = 0; r < PER; ++r) {
+            const uint32_t i = base + (uint32_t)(tid + r * nthreads);
+            v[r] = src[i < count ? i : count - 1];
+        }
+#pragma unroll
+        for (int r = 0; r < PER; ++r) {
+            const uint32_t i = base + (uint32_t)(tid + r * nthreads);
+            if (i < count) dst[i] = v[r];
+        }
+    }
+}
+
 template <int N>
 __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_SPEC_MIN_WAVES : 1) void k_spec_sparse(
     const uint32_t* __restrict__ row_start, const uint16_t* __restrict__ bucket,
@@ -336,7 +357,7 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
     const int n_rows = n_elems * N;
     const bool rows_lds = n_rows < SpecLds<N>::kRows;
     if (rows_lds)
-        for (int i = threadIdx.x; i <= n_rows; i += Cfg::BLOCK) lds_rows[i] = row_start[rbase + i];
+        stage_lds<4>(lds_rows, row_start + rbase, (uint32_t)n_rows + 1, threadIdx.x, Cfg::BLOCK);
     const bool tw_lds = n2 <= SpecLds<N>::kTwq;
     if (tw_lds)
         for (int i = threadIdx.x; i < n2; i += Cfg::BLOCK) lds_twq[i] = twq[i * tw_step];
@@ -355,8 +376,7 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
     const uint32_t kcount = kend - kbase;
     // everything in LDS (the usual case), or the bucket in pieces
     const bool all_lds = rows_lds && tw_lds && kcount <= (uint32_t)SpecLds<N>::kBucket;
-    if (all_lds)
-        for (uint32_t i = threadIdx.x; i < kcount; i += Cfg::BLOCK) lds_bucket[i] = bucket[kbase + i];
+    if (all_lds) stage_lds<4>(lds_bucket, bucket + kbase, kcount, threadIdx.x, Cfg::BLOCK);
     __syncthreads();
 
     float2 acc[EPT];
