@@ -1,9 +1,10 @@
 """CPU pins of the float32 algorithms the kernels run (emulated in numpy).
 
 fast_sincos (emsim_amd/csrc/fft_lds.hpp): Cody-Waite reduction by pi/2 with
-a three-part split and Taylor polynomials.  The emulation follows the device
-code step by step; FMA is emulated by one float64 operation rounded to
-float32 (exact for these operands up to a double rounding).
+a three-part split and fitted polynomials (sin r = r + r^3 p(r^2), cos r =
+1 - r^2/2 + r^4 q(r^2)).  The emulation follows the device code step by
+step; FMA is emulated by one float64 operation rounded to float32 (exact for
+these operands up to a double rounding).
 """
 import math
 import re
@@ -30,21 +31,12 @@ def fma(a, b, c):
 def fast_sincos(x):
     c = _constants()
     two_over_pi, p1, p2, p3 = (F(v) for v in c[0:4])
-    sin_coef, cos_coef = c[4:9], c[9:15]
     x = F(x)
     kf = F(np.rint(F(x * two_over_pi)))
     r = fma(-kf, p1, x)
     r = fma(-kf, p2, r)
     r = fma(-kf, p3, r)
-    r2 = F(r * r)
-    ps = F(sin_coef[0])
-    for v in sin_coef[1:]:
-        ps = fma(ps, r2, F(v))
-    sn = fma(F(ps * r2), r, r)
-    pc = F(cos_coef[0])
-    for v in cos_coef[1:]:
-        pc = fma(pc, r2, F(v))
-    cs = fma(pc, r2, F(1.0))
+    sn, cs = _poly(r)
     q = int(kf) & 3
     s, co = (cs, sn) if q & 1 else (sn, cs)
     if q in (1, 2):
@@ -54,16 +46,42 @@ def fast_sincos(x):
     return s, co
 
 
+def _poly(r):
+    """sin r and cos r on [-pi/4, pi/4], as the device evaluates them."""
+    c = _constants()
+    s3, s2, s1, c3, c2, c1, half, one = (F(v) for v in c[4:12])
+    r2 = F(r * r)
+    ps = fma(fma(s3, r2, s2), r2, s1)
+    sn = fma(ps, F(r * r2), r)
+    pc = fma(fma(c3, r2, c2), r2, c1)
+    cs = fma(pc, F(r2 * r2), fma(half, r2, one))
+    return sn, cs
+
+
 def test_constants_are_the_intended_ones():
     c = _constants()
     assert "fabsf(x) <= 8192.f" in SRC.read_text()
     assert abs(c[0] - 2 / math.pi) < 1e-16
     assert abs(c[1] + c[2] + c[3] - math.pi / 2) < 1e-14
-    # Taylor coefficients of sin (x^11 .. x^3) and cos (x^12 .. x^2)
-    for v, k in zip(c[4:9], (11, 9, 7, 5, 3)):
-        assert abs(abs(v) * math.factorial(k) - 1) < 1e-12
-    for v, k in zip(c[9:15], (12, 10, 8, 6, 4, 2)):
-        assert abs(abs(v) * math.factorial(k) - 1) < 1e-12
+    # the fitted coefficients stay close to the Taylor ones they refine
+    # (sin: x^7, x^5, x^3; cos: x^8, x^6, x^4), with the same signs
+    for v, k in zip(c[4:7], (7, 5, 3)):
+        assert (-1) ** ((k - 1) // 2) * v > 0 and abs(abs(v) * math.factorial(k) - 1) < 0.02
+    for v, k in zip(c[7:10], (8, 6, 4)):
+        assert (-1) ** (k // 2) * v > 0 and abs(abs(v) * math.factorial(k) - 1) < 0.02
+    assert c[10] == -0.5 and c[11] == 1.0
+
+
+def test_polynomials_within_1_5_ulp_on_the_reduced_interval():
+    r = np.linspace(-math.pi / 4, math.pi / 4, 200001).astype(F)
+    worst_s = worst_c = 0.0
+    for x in r[::7]:
+        sn, cs = _poly(x)
+        es, ec = math.sin(float(x)), math.cos(float(x))
+        if es != 0.0:
+            worst_s = max(worst_s, abs(float(sn) - es) / float(np.spacing(F(abs(es)))))
+        worst_c = max(worst_c, abs(float(cs) - ec) / float(np.spacing(F(abs(ec)))))
+    assert worst_s < 1.5 and worst_c < 1.5, (worst_s, worst_c)
 
 
 @pytest.mark.parametrize("lo,hi", [(-8192, 8192), (-10, 10), (0, 0.1)])
