@@ -176,7 +176,9 @@ __device__ __forceinline__ float2 tw_get(const TwLds<N>& c, int q, int r, int k)
 // per thread.  Before: v[m] = x[t + m N/EPT].  Butterfly j = t + q N/EPT
 // (q < EPT/R) reads x[j + r N/R] (= v[q + r EPT/R]) and writes
 // y[(j/Ns) Ns R + j%Ns + r Ns].
-template <int N, int EPT, int R, int Ns, bool INV, class TW>
+// SYNC2: the barrier after the reads, which keeps the next pass's writes off
+// planes still being read; a caller that alternates two plane sets drops it.
+template <int N, int EPT, int R, int Ns, bool INV, class TW, bool SYNC2 = true>
 __device__ __forceinline__ void stockham_pass(float2 (&v)[EPT], float* __restrict__ re,
                                               float* __restrict__ im, int t, const TW& tw) {
     constexpr int NB = EPT / R;
@@ -217,7 +219,7 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[EPT], float* __restric
         const int base = L::at(t);
 #pragma unroll
         for (int m = 0; m < EPT; ++m) v[m] = make_float2(re[base + L::at(m * S)], im[base + L::at(m * S)]);
-        __syncthreads();
+        if constexpr (SYNC2) __syncthreads();
     }
 }
 
@@ -228,6 +230,33 @@ __device__ __forceinline__ void fft_rest(float2 (&v)[EPT], float* re, float* im,
         stockham_pass<N, EPT, 8, Ns, INV>(v, re, im, t, tw);
         fft_rest<N, EPT, Ns * 8, INV>(v, re, im, t, tw);
     }
+}
+
+// LDS exchanges of one transform (every pass but the last writes one)
+constexpr int fft_exchanges(int n) { return (ilog2(n) - ilog2(lead_radix(n))) / 3; }
+
+template <int N, int EPT, int Ns, bool INV, int PASS, class TW>
+__device__ __forceinline__ void fft_rest_db(float2 (&v)[EPT], float* re0, float* im0, float* re1,
+                                            float* im1, int t, const TW& tw) {
+    if constexpr (Ns < N) {
+        stockham_pass<N, EPT, 8, Ns, INV, TW, false>(v, PASS % 2 ? re1 : re0,
+                                                     PASS % 2 ? im1 : im0, t, tw);
+        fft_rest_db<N, EPT, Ns * 8, INV, PASS + 1>(v, re0, im0, re1, im1, t, tw);
+    }
+}
+
+// fft_n with the exchanges alternating between two plane sets, one barrier
+// per exchange.  Safe for back-to-back transforms of the same block when
+// the exchange count is even: a thread writes a plane set only after the
+// barrier of the exchange that followed its last use, which every thread
+// passes only once its reads of that set are done.
+template <int N, int EPT, bool INV, class TW>
+__device__ __forceinline__ void fft_n_db(float2 (&v)[EPT], float* re0, float* im0, float* re1,
+                                         float* im1, int t, const TW& tw) {
+    static_assert(fft_exchanges(N) % 2 == 0, "alternating planes need an even exchange count");
+    constexpr int R0 = lead_radix(N);
+    stockham_pass<N, EPT, R0, 1, INV, TW, false>(v, re0, im0, t, tw);
+    fft_rest_db<N, EPT, R0, INV, 1>(v, re0, im0, re1, im1, t, tw);
 }
 
 // Unnormalised length-N DFT of the values held by the N/EPT threads of one

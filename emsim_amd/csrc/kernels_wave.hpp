@@ -327,6 +327,11 @@ __device__ __forceinline__ int64_t col_tile(unsigned L, int tpb, int r) {
     const int64_t pair = (int64_t)(k / 2) * kColXcds + x;
     return (pair * tpb + r) * 2 + (k % 2);
 }
+#ifndef EMSIM_COL_DB
+#define EMSIM_COL_DB 1
+#endif
+constexpr bool kColDoubleBuffer = EMSIM_COL_DB != 0;
+
 // Wave columns j in [lo, hi] that the propagator zeroes whole (none: lo >
 // hi).  Column tiles skip_lo .. skip_lo + skip_n - 1 of each image hold
 // only such columns; the launch runs the other live_per_img tiles.
@@ -337,19 +342,32 @@ struct DeadCols {
     __device__ __forceinline__ int tile(int tl) const { return tl < skip_lo ? tl : tl + skip_n; }
 };
 
+// lengths whose column transforms alternate two plane sets; their kernels
+// are held to 128 VGPRs (two 8-wave blocks per CU), which the second set's
+// addresses would otherwise push past
+template <int N>
+constexpr bool col_db() { return kColDoubleBuffer && fft_exchanges(N) % 2 == 0 && N >= 64; }
+
 template <int N, class Mul>
-__global__ __launch_bounds__(ColCfg<N>::BLOCK, EMSIM_COL_MIN_WAVES) void k_col_fft_mul_ifft(
+__global__ __launch_bounds__(ColCfg<N>::BLOCK, col_db<N>() ? 4 : EMSIM_COL_MIN_WAVES) void k_col_fft_mul_ifft(
     float2* data, int ld, int ncols, DeadCols dc, int64_t img_stride, int64_t n_tiles,
     int tpb, Mul mul, const float2* __restrict__ tw) {
     using Cfg = ColCfg<N>;
     constexpr int EPT = Cfg::EPT;
+    // a second plane set where the exchange count allows alternating sets
+    // (one barrier per exchange instead of two)
+    constexpr bool DB = col_db<N>();
     __shared__ float lds_re[Cfg::C * Cfg::STRIDE];
     __shared__ float lds_im[Cfg::C * Cfg::STRIDE];
+    __shared__ float lds_re2[DB ? Cfg::C * Cfg::STRIDE : 1];
+    __shared__ float lds_im2[DB ? Cfg::C * Cfg::STRIDE : 1];
     __shared__ float2 lds_px[Mul::kTable ? N : 1];
     const int c = threadIdx.x % Cfg::C;
     const int t = threadIdx.x / Cfg::C;
     float* re = lds_re + c * Cfg::STRIDE;
     float* im = lds_im + c * Cfg::STRIDE;
+    float* re2 = lds_re2 + (DB ? c * Cfg::STRIDE : 0);
+    float* im2 = lds_im2 + (DB ? c * Cfg::STRIDE : 0);
     mul.prepare(lds_px, threadIdx.x, Cfg::BLOCK);
     if constexpr (Mul::kTable) __syncthreads();
 
@@ -386,10 +404,12 @@ __global__ __launch_bounds__(ColCfg<N>::BLOCK, EMSIM_COL_MIN_WAVES) void k_col_f
                 nxt[m] = live_n ? base_n[(int64_t)(t + m * Cfg::T) * ld] : make_float2(0.f, 0.f);
         }
         const float2 py = mul.column(col);
-        fft_n<N, EPT, false>(cur, re, im, t, twc);
+        if constexpr (DB) fft_n_db<N, EPT, false>(cur, re, im, re2, im2, t, twc);
+        else fft_n<N, EPT, false>(cur, re, im, t, twc);
 #pragma unroll
         for (int m = 0; m < EPT; ++m) cur[m] = mul.apply(cur[m], t + m * Cfg::T, col, lds_px, py);
-        fft_n<N, EPT, true>(cur, re, im, t, twc);
+        if constexpr (DB) fft_n_db<N, EPT, true>(cur, re, im, re2, im2, t, twc);
+        else fft_n<N, EPT, true>(cur, re, im, t, twc);
         if (ok) {
 #pragma unroll
             for (int m = 0; m < EPT; ++m) base[(int64_t)(t + m * Cfg::T) * ld] = cur[m];
