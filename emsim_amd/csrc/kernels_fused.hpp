@@ -674,13 +674,20 @@ struct OpSlicePassA {
             dst[N + t + m * T] = vb[m];
         }
     }
-    __device__ __forceinline__ void store_w(int64_t pair, int row, int t, bool ok,
+    // bit m: this thread's element m is a live column (the same for every row)
+    __device__ __forceinline__ uint32_t live_mask(int t) const {
+        uint32_t bits = 0;
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) bits |= dead(t + m * T) ? 0u : (1u << m);
+        return bits;
+    }
+    __device__ __forceinline__ void store_w(int64_t pair, int row, int t, bool ok, uint32_t live,
                                             const float2 (&w)[EPT]) const {
         if (!ok) return;
         float2* dst = wave + (pair * 2 + row) * (int64_t)N;
 #pragma unroll
         for (int m = 0; m < EPT; ++m)
-            if (!dead(t + m * T)) dst[t + m * T] = w[m];
+            if (live & (1u << m)) dst[t + m * T] = w[m];
     }
 };
 
@@ -709,6 +716,7 @@ __global__ __launch_bounds__(RowCfg<N>::BLOCK, EMSIM_PASSA_MIN_WAVES) void k_sli
     float2 ya[EPT], yb[EPT], w0[EPT], w1[EPT];
     op.load_y(first, t, first < n_pairs, ya, yb);
     if constexpr (!FIRST) op.load_w(first, 0, t, first < n_pairs, w0);
+    const uint32_t live = op.live_mask(t);
     twc.fence();
     for (int r = 0; r < rpw; ++r) {
         const int64_t it = first + (int64_t)r * Cfg::G;
@@ -725,14 +733,14 @@ __global__ __launch_bounds__(RowCfg<N>::BLOCK, EMSIM_PASSA_MIN_WAVES) void k_sli
         }
         op.store_v(it, t, ok, va, vb);
         op.transmit_row(w0, va, t, re, im, twc);
-        op.store_w(it, 0, t, ok, w0);
+        op.store_w(it, 0, t, ok, live, w0);
         if (r + 1 < rpw) {
             const int64_t nx = it + Cfg::G;
             op.load_y(nx, t, nx < n_pairs, ya, yb);
             if constexpr (!FIRST) op.load_w(nx, 0, t, nx < n_pairs, w0);
         }
         op.transmit_row(w1, vb, t, re, im, twc);
-        op.store_w(it, 1, t, ok, w1);
+        op.store_w(it, 1, t, ok, live, w1);
     }
 }
 
