@@ -12,6 +12,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "../../include/emsim_hip.h"
@@ -1067,7 +1068,7 @@ template <int N>
 int launch_spec_sparse(const uint32_t* row_start, const uint16_t* bucket, const int2* twq,
                        const float* sf, float2* Y, int slice, int n_slices, int n_elems, int H,
                        int ldh, int n2, int bandlimit, int band_r2, float inv_npix, int batch,
-                       const float2* tw, hipStream_t s) {
+                       const int* order, const float2* tw, hipStream_t s) {
     using Cfg = SpecCfg<N>;
     if (H != n2 / 2 + 1 || n2 < 2 || ldh < H + n_elems)
         return fail(EMSIM_HIP_EINVAL, "spectrum: needs H = n2/2 + 1, n2 >= 2, ldh >= H + n_elems");
@@ -1076,7 +1077,7 @@ int launch_spec_sparse(const uint32_t* row_start, const uint16_t* bucket, const 
     const dim3 grid((unsigned)(ceil_div(cols, Cfg::C) * (int64_t)batch));
     EMSIM_LAUNCH(N, (k_spec_sparse<N>), grid, dim3(Cfg::BLOCK), 0, s, row_start, bucket, twq, sf,
                  Y, slice, n_slices, n_elems, H, ldh, n2, batch, bandlimit, band_r2, inv_npix,
-                 tw);
+                 order, tw);
     return EMSIM_HIP_OK;
 }
 
@@ -1111,10 +1112,10 @@ int launch_col_lensprop(float2* data, int ld, int ncols, int64_t img_stride, int
 
 int spec_sparse(int n, const uint32_t* row_start, const uint16_t* bucket, const int2* twq,
                 const float* sf, float2* Y, int slice, int n_slices, int n_elems, int H, int ldh,
-                int n2, int bandlimit, int band_r2, float inv_npix, int batch, const float2* tw,
-                hipStream_t s) {
+                int n2, int bandlimit, int band_r2, float inv_npix, int batch, const int* order,
+                const float2* tw, hipStream_t s) {
     EMSIM_DISPATCH(n, launch_spec_sparse, row_start, bucket, twq, sf, Y, slice, n_slices,
-                   n_elems, H, ldh, n2, bandlimit, band_r2, inv_npix, batch, tw, s);
+                   n_elems, H, ldh, n2, bandlimit, band_r2, inv_npix, batch, order, tw, s);
 }
 int slice_pass_a(int n, bool first, const float2* Y, float2* wave, int64_t n_pairs, int ldh,
                  float psi0, float factor, float* vout, int slice, int n_slices, int n1,
@@ -1128,6 +1129,7 @@ int col_lensprop(int n, float2* data, int ld, int ncols, int64_t img_stride, int
 }
 
 struct FusedWorkspace {
+    int32_t* order;   // molecule slots of the spectrum pass, or null (identity)
     int64_t* mol_off;
     int64_t* elem_off;
     int32_t* atom_row;
@@ -1150,6 +1152,7 @@ FusedWorkspace fused_layout(const emsim_hip_slice_builder* b, int batch, int64_t
     const int64_t n_blocks = ceil_div(n_scan, kScanTile);
     const int ldh = fused_ld(b);
     size_t off = 0;
+    w.order = (int32_t*)(base + off);      off += align_up(sizeof(int32_t) * (size_t)batch);
     w.mol_off = (int64_t*)(base + off);    off += align_up(sizeof(int64_t) * (size_t)(batch + 1));
     w.elem_off = (int64_t*)(base + off);
     off += align_up(sizeof(int64_t) * (size_t)batch * (b->n_elems + 1));
@@ -1243,6 +1246,14 @@ extern "C" int emsim_hip_simulate(const emsim_hip_slice_builder* b,
                                    hipMemcpyHostToDevice, s));
     EMSIM_HIP_CHECK(hipMemcpyAsync(ws.elem_off, elem_off.data(), sizeof(int64_t) * elem_off.size(),
                                    hipMemcpyHostToDevice, s));
+    // spectrum-pass molecule slots by descending atom count (ties by index)
+    std::vector<int32_t> order((size_t)batch);
+    for (int m = 0; m < batch; ++m) order[(size_t)m] = m;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t c) {
+        return mol_off_host[a + 1] - mol_off_host[a] > mol_off_host[c + 1] - mol_off_host[c];
+    });
+    EMSIM_HIP_CHECK(hipMemcpyAsync(ws.order, order.data(), sizeof(int32_t) * order.size(),
+                                   hipMemcpyHostToDevice, s));
     return simulate_core(b, w, coords, n_atoms, ws, batch, psi0, cs_mm, defocus, aperture,
                          symmetric_bandlimit, out, slices_out, mode, s);
 }
@@ -1263,6 +1274,7 @@ extern "C" int emsim_hip_simulate_dev(const emsim_hip_slice_builder* b,
         return fail(EMSIM_HIP_EWORKSPACE, "simulate: workspace needs %zu bytes", ws.bytes);
     ws.mol_off = const_cast<int64_t*>(mol_off_dev);
     ws.elem_off = const_cast<int64_t*>(elem_off_dev);
+    ws.order = nullptr;   // counts are on the device only: identity slots
     return simulate_core(b, w, coords, n_atoms, ws, batch, psi0, cs_mm, defocus, aperture,
                          symmetric_bandlimit, out, slices_out, mode, (hipStream_t)stream_);
 }
@@ -1338,7 +1350,7 @@ int simulate_core(const emsim_hip_slice_builder* b, const emsim_hip_wave_propaga
         const bool last = sl == b->n_slices - 1;
         rc = spec_sparse(b->n1, ws.row_start, ws.bucket, twq, b->sf, ws.Y, sl, b->n_slices,
                          b->n_elems, b->H, ldh, b->n2, symmetric_bandlimit ? 1 : 0, r * r,
-                         inv_npix, batch, b->tw, s);
+                         inv_npix, batch, ws.order, b->tw, s);
         if (rc) return rc;
         rc = slice_pass_a(b->n2, sl == 0, ws.Y, wave, rows / 2, ldh, psi0, w->factor, slices_out,
                           sl, b->n_slices, b->n1, dead, b->tw, s);

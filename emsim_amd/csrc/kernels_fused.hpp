@@ -318,7 +318,7 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
     const uint32_t* __restrict__ row_start, const uint16_t* __restrict__ bucket,
     const int2* __restrict__ twq, const float* __restrict__ sf, float2* __restrict__ Y,
     int slice, int n_slices, int n_elems, int H, int ldh, int n2, int batch, int bandlimit,
-    int band_r2, float inv_npix, const float2* __restrict__ tw) {
+    int band_r2, float inv_npix, const int* __restrict__ order, const float2* __restrict__ tw) {
     using Cfg = SpecCfg<N>;
     constexpr int EPT = Cfg::EPT;
     __shared__ float lds_re[Cfg::C * Cfg::STRIDE];
@@ -334,7 +334,11 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
     __shared__ float2 lds_tw[kTwInLds ? N : 1];
     const unsigned item = spec_item(blockIdx.x, gridDim.x);
     const int tile = (int)(item / (unsigned)batch);
-    const int b = (int)(item % (unsigned)batch);
+    // molecule slot -> molecule (null: identity).  Slots in order of atom
+    // count keep neighbouring work items alike in cost; the XCDs, whose item
+    // ranges start at different slots, then progress at similar rates.
+    const int bi = (int)(item % (unsigned)batch);
+    const int b = order ? __builtin_amdgcn_readfirstlane(order[bi]) : bi;
     const int c = threadIdx.x % Cfg::C;
     const int t = threadIdx.x / Cfg::C;
     const int col = tile * Cfg::C + c;
