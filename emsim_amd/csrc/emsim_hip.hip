@@ -12,7 +12,6 @@
 #include <mutex>
 #include <new>
 #include <string>
-#include <algorithm>
 #include <vector>
 
 #include "../../include/emsim_hip.h"
@@ -1129,7 +1128,7 @@ int col_lensprop(int n, float2* data, int ld, int ncols, int64_t img_stride, int
 }
 
 struct FusedWorkspace {
-    int32_t* order;   // molecule slots of the spectrum pass, or null (identity)
+    int32_t* order;   // spectrum-pass molecule slots per slice, or null (identity)
     int64_t* mol_off;
     int64_t* elem_off;
     int32_t* atom_row;
@@ -1152,7 +1151,8 @@ FusedWorkspace fused_layout(const emsim_hip_slice_builder* b, int batch, int64_t
     const int64_t n_blocks = ceil_div(n_scan, kScanTile);
     const int ldh = fused_ld(b);
     size_t off = 0;
-    w.order = (int32_t*)(base + off);      off += align_up(sizeof(int32_t) * (size_t)batch);
+    w.order = (int32_t*)(base + off);
+    off += align_up(sizeof(int32_t) * (size_t)batch * b->n_slices);
     w.mol_off = (int64_t*)(base + off);    off += align_up(sizeof(int64_t) * (size_t)(batch + 1));
     w.elem_off = (int64_t*)(base + off);
     off += align_up(sizeof(int64_t) * (size_t)batch * (b->n_elems + 1));
@@ -1246,14 +1246,6 @@ extern "C" int emsim_hip_simulate(const emsim_hip_slice_builder* b,
                                    hipMemcpyHostToDevice, s));
     EMSIM_HIP_CHECK(hipMemcpyAsync(ws.elem_off, elem_off.data(), sizeof(int64_t) * elem_off.size(),
                                    hipMemcpyHostToDevice, s));
-    // spectrum-pass molecule slots by descending atom count (ties by index)
-    std::vector<int32_t> order((size_t)batch);
-    for (int m = 0; m < batch; ++m) order[(size_t)m] = m;
-    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t c) {
-        return mol_off_host[a + 1] - mol_off_host[a] > mol_off_host[c + 1] - mol_off_host[c];
-    });
-    EMSIM_HIP_CHECK(hipMemcpyAsync(ws.order, order.data(), sizeof(int32_t) * order.size(),
-                                   hipMemcpyHostToDevice, s));
     return simulate_core(b, w, coords, n_atoms, ws, batch, psi0, cs_mm, defocus, aperture,
                          symmetric_bandlimit, out, slices_out, mode, s);
 }
@@ -1274,7 +1266,6 @@ extern "C" int emsim_hip_simulate_dev(const emsim_hip_slice_builder* b,
         return fail(EMSIM_HIP_EWORKSPACE, "simulate: workspace needs %zu bytes", ws.bytes);
     ws.mol_off = const_cast<int64_t*>(mol_off_dev);
     ws.elem_off = const_cast<int64_t*>(elem_off_dev);
-    ws.order = nullptr;   // counts are on the device only: identity slots
     return simulate_core(b, w, coords, n_atoms, ws, batch, psi0, cs_mm, defocus, aperture,
                          symmetric_bandlimit, out, slices_out, mode, (hipStream_t)stream_);
 }
@@ -1324,6 +1315,14 @@ int simulate_core(const emsim_hip_slice_builder* b, const emsim_hip_wave_propaga
     if (n_atoms > 0)
         EMSIM_LAUNCH(0, k_csr_scatter, dim3(grid_for(n_atoms, 256)), dim3(256), 0, s, n_atoms,
                      ws.atom_row, ws.atom_y, ws.row_start, ws.row_count, ws.bucket);
+    // per-slice molecule order of the spectrum pass (identity beyond one
+    // block's worth of molecules)
+    const int32_t* order = nullptr;
+    if (batch > 1 && batch <= 1024) {
+        EMSIM_LAUNCH(0, k_slice_order, dim3((unsigned)b->n_slices), dim3((unsigned)batch), 0, s,
+                     ws.row_start, b->n_slices, b->n_elems, b->n1, batch, ws.order);
+        order = ws.order;
+    }
 
     // 2. the slices
     float2* wave = mode == 0 ? ws.wave : (float2*)out;
@@ -1350,7 +1349,7 @@ int simulate_core(const emsim_hip_slice_builder* b, const emsim_hip_wave_propaga
         const bool last = sl == b->n_slices - 1;
         rc = spec_sparse(b->n1, ws.row_start, ws.bucket, twq, b->sf, ws.Y, sl, b->n_slices,
                          b->n_elems, b->H, ldh, b->n2, symmetric_bandlimit ? 1 : 0, r * r,
-                         inv_npix, batch, ws.order, b->tw, s);
+                         inv_npix, batch, order, b->tw, s);
         if (rc) return rc;
         rc = slice_pass_a(b->n2, sl == 0, ws.Y, wave, rows / 2, ldh, psi0, w->factor, slices_out,
                           sl, b->n_slices, b->n1, dead, b->tw, s);

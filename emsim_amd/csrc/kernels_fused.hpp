@@ -89,6 +89,31 @@ __global__ void k_csr_scatter(int64_t n_atoms, const int32_t* __restrict__ atom_
     }
 }
 
+// Spectrum-pass molecule slots of every slice: slot r of slice s is the
+// molecule with the r-th most atoms in that slice (ties by index).  One
+// block per slice, one thread per molecule (batch <= blockDim.x); a
+// (molecule, slice)'s rows of all elements are adjacent in the row table.
+__global__ void k_slice_order(const uint32_t* __restrict__ row_start, int n_slices, int n_elems,
+                              int n1, int batch, int32_t* __restrict__ order) {
+    __shared__ uint32_t cnt[1024];
+    const int s = blockIdx.x, i = threadIdx.x;
+    const int64_t rows = (int64_t)n_elems * n1;
+    if (i < batch) {
+        const int64_t r0 = ((int64_t)i * n_slices + s) * rows;
+        cnt[i] = row_start[r0 + rows] - row_start[r0];
+    }
+    __syncthreads();
+    if (i < batch) {
+        const uint32_t c = cnt[i];
+        int rank = 0;
+        for (int j = 0; j < batch; ++j) {
+            const uint32_t o = cnt[j];
+            rank += (o > c || (o == c && j < i)) ? 1 : 0;
+        }
+        order[(int64_t)s * batch + rank] = i;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Spectrum of one slice of every molecule, from the atom buckets.
 // One block per (column tile, molecule): a 1-D grid of n_tiles * batch.
@@ -334,11 +359,11 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
     __shared__ float2 lds_tw[kTwInLds ? N : 1];
     const unsigned item = spec_item(blockIdx.x, gridDim.x);
     const int tile = (int)(item / (unsigned)batch);
-    // molecule slot -> molecule (null: identity).  Slots in order of atom
-    // count keep neighbouring work items alike in cost; the XCDs, whose item
-    // ranges start at different slots, then progress at similar rates.
+    // molecule slot -> molecule (null: identity).  Slots in order of the
+    // slice's atom count, so each XCD's stream of blocks within a column
+    // tile runs from the most expensive molecules to the cheapest.
     const int bi = (int)(item % (unsigned)batch);
-    const int b = order ? __builtin_amdgcn_readfirstlane(order[bi]) : bi;
+    const int b = order ? __builtin_amdgcn_readfirstlane(order[(int64_t)slice * batch + bi]) : bi;
     const int c = threadIdx.x % Cfg::C;
     const int t = threadIdx.x / Cfg::C;
     const int col = tile * Cfg::C + c;
