@@ -163,6 +163,36 @@ def test_dense_slices_match_oracle():
     assert ei < TOL and ex < TOL and es < 2e-6
 
 
+def test_bench_workload_images_match_port():
+    """The benchmark's own configuration -- 512^2, 100 slices of 1 A, its batch
+    of 64 molecules (per-slice spectrum order, the propagator's dead columns
+    left out between passes) -- against oracle/fftw_port.c for the largest
+    and the smallest molecule of the batch."""
+    port = pytest.importorskip("oracle.port")
+    try:
+        port.load()
+    except Exception as exc:
+        pytest.fail(f"oracle/fftw_port is not built: {exc}")
+    import bench
+    tem = emsim_amd.tem.TEM(electron_dose=20, beam_energy_kev=200, cs_mm=1.3, defocus=700,
+                            aperture=10e-3)
+    elements, n, ns, d, dz = [1, 6, 7, 8, 16], 512, 100, 1.0, 1.0
+    mols, _ = bench.workload(64)
+    pipe = BatchPipe(tem, d, dz, n, ns, elements)
+    images = pipe.run(pipe.upload([pipe.prepare(m) for m in mols])).cpu().numpy()
+    lam, gamma = orc.wavelength_and_gamma(tem.beam_energy_kev)
+    psi0 = float(np.float32(np.sqrt(tem.electron_dose * d * d)))
+    sf = orc.scattering_factors_half(elements, n, n, d, d)
+    sizes = [len(m) for m in mols]
+    for i in (int(np.argmax(sizes)), int(np.argmin(sizes))):
+        mol, _, counts = atm.sort_elements_and_count(atm.centralize(mols[i]), elements)
+        ref = port.simulate(mol.coordinates, counts, ns, n, dz, d, sf, lam, gamma, tem.cs_mm,
+                            tem.defocus, tem.aperture, psi0)
+        e = orc.rel_l2(images[i], ref)
+        print(f"bench molecule {i} ({sizes[i]} atoms): image rel L2 vs fftw_port {e:.2e}")
+        assert e < TOL
+
+
 def test_batch_sim_delivers_in_order_and_skips_failures():
     """The batched simulation loop: batches of 3 over 7 molecules (the last
     batch partial), images through the pinned ring equal to one fused run of
