@@ -105,6 +105,8 @@ _SIGNATURES = {
     "emsim_hip_orient_workspace_size": (c_size_t, [c_int]),
     "emsim_hip_orient_batch": (c_int, [c_void_p, c_int64, c_void_p, c_float, c_float, c_float,
                                        c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "emsim_hip_orient_batch_noalloc": (c_int, [c_void_p, c_int64, c_void_p, c_float, c_float,
+                                               c_float, c_int, c_void_p, c_void_p]),
     "emsim_hip_tuning_set": (c_int, [c_char_p, c_int]),
     "emsim_hip_tuning_get": (c_int, [c_char_p]),
     "emsim_hip_timing_collect": (c_int, [c_void_p, c_size_t, POINTER(c_size_t)]),

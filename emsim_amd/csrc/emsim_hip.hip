@@ -957,6 +957,22 @@ extern "C" size_t emsim_hip_orient_workspace_size(int batch) {
     return batch < 1 ? 0 : align_up(sizeof(uint32_t) * 6 * (size_t)batch);
 }
 
+namespace {
+// The three launches of an orientation batch; bbox holds 6 keys per instance
+// (device memory, cleared here by a fill kernel before use).
+int orient_launch(const float* base, int64_t n_atoms, const float* rot, float c0x, float c0y,
+                  float c0z, int batch, float* out, uint32_t* bbox, hipStream_t s) {
+    EMSIM_LAUNCH(0, k_bbox_init, dim3((unsigned)ceil_div(6 * (int64_t)batch, 256)), dim3(256), 0,
+                 s, bbox, batch);
+    EMSIM_LAUNCH(0, k_orient, dim3((unsigned)ceil_div(n_atoms, 256), (unsigned)batch), dim3(256),
+                 0, s, base, n_atoms, rot, c0x, c0y, c0z, out, bbox);
+    const int64_t total = (int64_t)batch * n_atoms * 3;
+    EMSIM_LAUNCH(0, k_recenter, dim3(grid_for(total, 256)), dim3(256), 0, s, out, n_atoms, batch,
+                 (const uint32_t*)bbox);
+    return EMSIM_HIP_OK;
+}
+}  // namespace
+
 extern "C" int emsim_hip_orient_batch(const float* base, int64_t n_atoms, const float* rot,
                                       float c0x, float c0y, float c0z, int batch, float* out,
                                       void* workspace, size_t workspace_bytes,
@@ -966,16 +982,55 @@ extern "C" int emsim_hip_orient_batch(const float* base, int64_t n_atoms, const 
     if (batch > 65535) return fail(EMSIM_HIP_EUNSUPPORTED, "orient_batch: batch <= 65535");
     if (workspace_bytes < emsim_hip_orient_workspace_size(batch))
         return fail(EMSIM_HIP_EWORKSPACE, "orient_batch: workspace too small");
-    hipStream_t s = (hipStream_t)stream_;
-    uint32_t* bbox = (uint32_t*)workspace;
-    EMSIM_LAUNCH(0, k_bbox_init, dim3((unsigned)ceil_div(6 * (int64_t)batch, 256)), dim3(256), 0,
-                 s, bbox, batch);
-    EMSIM_LAUNCH(0, k_orient, dim3((unsigned)ceil_div(n_atoms, 256), (unsigned)batch), dim3(256),
-                 0, s, base, n_atoms, rot, c0x, c0y, c0z, out, bbox);
-    const int64_t total = (int64_t)batch * n_atoms * 3;
-    EMSIM_LAUNCH(0, k_recenter, dim3(grid_for(total, 256)), dim3(256), 0, s, out, n_atoms, batch,
-                 (const uint32_t*)bbox);
+    return orient_launch(base, n_atoms, rot, c0x, c0y, c0z, batch, out, (uint32_t*)workspace,
+                         (hipStream_t)stream_);
+}
+
+// --------------------------------------------------------------------------
+// the same without a workspace argument, for callers that cannot allocate
+// --------------------------------------------------------------------------
+// The bounding-box keys live in a library-owned device array.  In host code
+// `scratch` only names the symbol; its device address (one copy per GPU) comes
+// from hipGetSymbolAddress, taken once per device with that device current.
+static __device__ __align__(256) uint32_t scratch[6 * EMSIM_HIP_ORIENT_NOALLOC_MAX_BATCH];
+
+namespace {
+std::mutex g_scratch_mutex;
+std::vector<uint32_t*> g_scratch_ptr;   // per device; null until first use
+
+int scratch_address(uint32_t** out) {
+    int dev = 0;
+    EMSIM_HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (err != hipSuccess || !p)
+            return fail(EMSIM_HIP_EDEVICE, "orient_batch_noalloc: hipGetSymbolAddress failed: %s",
+                        hipGetErrorString(err));
+        g_scratch_ptr[dev] = (uint32_t*)p;
+    }
+    *out = g_scratch_ptr[dev];
     return EMSIM_HIP_OK;
+}
+}  // namespace
+
+extern "C" int emsim_hip_orient_batch_noalloc(const float* base, int64_t n_atoms,
+                                              const float* rot, float c0x, float c0y, float c0z,
+                                              int batch, float* out, emsim_stream_t stream_) {
+    if (!base || !rot || !out || n_atoms < 1 || batch < 1)
+        return fail(EMSIM_HIP_EINVAL, "orient_batch_noalloc: bad arguments");
+    if (batch > EMSIM_HIP_ORIENT_NOALLOC_MAX_BATCH)
+        return fail(EMSIM_HIP_EUNSUPPORTED,
+                    "orient_batch_noalloc: batch %d exceeds the library's scratch (at most %d "
+                    "orientations per call; use emsim_hip_orient_batch with a workspace)",
+                    batch, EMSIM_HIP_ORIENT_NOALLOC_MAX_BATCH);
+    uint32_t* bbox = nullptr;
+    const int rc = scratch_address(&bbox);
+    if (rc) return rc;
+    return orient_launch(base, n_atoms, rot, c0x, c0y, c0z, batch, out, bbox,
+                         (hipStream_t)stream_);
 }
 
 // ==========================================================================

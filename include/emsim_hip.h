@@ -279,6 +279,21 @@ int emsim_hip_fno_force_generic(int on);
  * a training forward pass with the same x_dev and workspace. */
 int emsim_hip_fno_backward(emsim_hip_fno* f, const void* x_dev, const float* dout_dev, int batch,
                            void* workspace_dev, size_t workspace_bytes, emsim_stream_t stream);
+
+/* emsim_hip_orient_batch without a workspace argument; it allocates nothing.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops).  The
+ * bounding-box keys live in a library-owned device array (96 KiB on each GPU
+ * the library runs on, held until the process ends), so batch is at most
+ * EMSIM_HIP_ORIENT_NOALLOC_MAX_BATCH (larger batches: EMSIM_HIP_EUNSUPPORTED).
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (concurrent streams, several host threads) race and must be serialised
+ * by the caller.  The first call on a device looks up the array's address
+ * (hipGetSymbolAddress), so make it once outside graph capture. */
+#define EMSIM_HIP_ORIENT_NOALLOC_MAX_BATCH 4096
+int emsim_hip_orient_batch_noalloc(const float* base_dev, int64_t n_atoms, const float* rot_dev,
+                                   float c0x, float c0y, float c0z, int batch, float* out_dev,
+                                   emsim_stream_t stream);
 int emsim_hip_fno_adam(emsim_hip_fno* f, float lr, float beta1, float beta2, float eps,
                        float weight_decay, int step, emsim_stream_t stream);
 /* wave_dev = coarse_wave_dev + psi0 (out_r + i out_i), complex64. */
