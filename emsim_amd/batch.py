@@ -132,16 +132,28 @@ class GraphRun:
 
 
 class BatchPipe:
+    """Batched form of ``Pipe`` for molecules whose atoms are uploaded once.
+
+    ``add_water=True`` is the reference's default image (``pipe.py:16``):
+    every voxel empty of all elements gets Poisson water
+    (``slice_builder_base.py:134-147``).  Each molecule draws from its own
+    seed, the value ``MultiSlicesBuilder.add_water`` would take for it:
+    given per call, else drawn from numpy's global RNG, one per molecule in
+    batch order."""
+
     def __init__(self, microscope: TEM, pixel_size: float, slice_thickness: float,
                  roi: int, n_slices: int, elements: Sequence[int] = (1, 6, 7, 8, 16),
-                 symmetric_bandlimit: bool = True):
+                 symmetric_bandlimit: bool = True, add_water: bool = False):
         self.microscope = microscope
+        self.add_water = bool(add_water)
         self.pixel_size = float(pixel_size)
         self.dz = float(slice_thickness)
         self.n = int(roi)
         self.n_slices = int(n_slices)
         self.elements = sorted(int(z) for z in elements)
         self.symmetric_bandlimit = symmetric_bandlimit
+        if self.add_water and (1 not in self.elements or 8 not in self.elements):
+            raise ValueError("add_water needs H (Z=1) and O (Z=8) among the pipe's elements")
         self.builder = MultiSlicesBuilder(self.elements, self.n_slices, self.n, self.n, self.dz,
                                           self.pixel_size, self.pixel_size)
         self.propagator = WavePropagator(self.n, self.n, self.pixel_size, self.pixel_size,
@@ -217,30 +229,53 @@ class BatchPipe:
         return t
 
     # -- fused GPU path --------------------------------------------------------
-    def _simulate(self, batch: PreparedBatch, mode: int, slices_out=None) -> torch.Tensor:
+    def water_seeds(self, batch: PreparedBatch, seeds=None) -> np.ndarray:
+        """(B,) uint64 water seeds: the given ones, else one draw per molecule
+        from numpy's global RNG (as ``MultiSlicesBuilder.add_water`` draws)."""
+        if seeds is None:
+            seeds = [int(np.random.randint(0, 2 ** 63 - 1, dtype=np.int64))
+                     for _ in range(len(batch))]
+        seeds = np.ascontiguousarray(np.asarray(seeds, dtype=np.uint64).reshape(-1))
+        if seeds.size != len(batch):
+            raise ValueError(f"need one water seed per molecule ({len(batch)}), got {seeds.size}")
+        return seeds
+
+    def _simulate(self, batch: PreparedBatch, mode: int, slices_out=None,
+                  seeds=None) -> torch.Tensor:
+        from ctypes import c_uint64
+        from .physics import water_num_dens
         B = len(batch)
         n_atoms = batch.coords.shape[0]
         if mode == 0:
             out = torch.empty((B, self.n, self.n), dtype=torch.float32, device=device())
         else:
             out = torch.empty((B, self.n, self.n), dtype=torch.complex64, device=device())
-        ws_bytes = hip_ext.raw("emsim_hip_simulate_workspace_size")(
+        water = "_water" if self.add_water else ""
+        ws_bytes = hip_ext.raw(f"emsim_hip_simulate{water}_workspace_size")(
             self.builder.backend.h, B, n_atoms, mode)
         ws = self._buffer("simulate_ws", (max(ws_bytes, 1),), torch.uint8)
         mic = self.microscope
         psi0 = self.propagator.init_wave_value(mic.electron_dose)
-        hip_ext.call("emsim_hip_simulate", self.builder.backend.h, self.propagator.backend.h,
-                     ptr(batch.coords), n_atoms, batch.mol_off.ctypes.data_as(POINTER(c_int64)),
-                     batch.counts.ctypes.data_as(POINTER(c_int64)), B, float(psi0),
-                     float(mic.cs_mm), float(mic.defocus), float(mic.aperture),
-                     1 if self.symmetric_bandlimit else 0, ptr(out),
-                     None if slices_out is None else ptr(slices_out), mode, ptr(ws), ws.numel(),
-                     stream_handle())
+        head = (self.builder.backend.h, self.propagator.backend.h, ptr(batch.coords), n_atoms,
+                batch.mol_off.ctypes.data_as(POINTER(c_int64)),
+                batch.counts.ctypes.data_as(POINTER(c_int64)), B, float(psi0), float(mic.cs_mm),
+                float(mic.defocus), float(mic.aperture), 1 if self.symmetric_bandlimit else 0)
+        tail = (ptr(out), None if slices_out is None else ptr(slices_out), mode, ptr(ws),
+                ws.numel(), stream_handle())
+        if not self.add_water:
+            hip_ext.call("emsim_hip_simulate", *head, *tail)
+            return out
+        seeds = self.water_seeds(batch, seeds)
+        voxel_water = water_num_dens * self.pixel_size * self.pixel_size * self.dz
+        hip_ext.call("emsim_hip_simulate_water", *head, self.elements.index(1),
+                     self.elements.index(8), float(voxel_water),
+                     seeds.ctypes.data_as(POINTER(c_uint64)), *tail)
         return out
 
-    def run(self, batch: PreparedBatch) -> torch.Tensor:
-        """(B, n, n) float32 images on the GPU."""
-        return self._simulate(batch, 0)
+    def run(self, batch: PreparedBatch, seeds=None) -> torch.Tensor:
+        """(B, n, n) float32 images on the GPU (``seeds``: water seeds, used
+        when the pipe adds water)."""
+        return self._simulate(batch, 0, seeds=seeds)
 
     # -- captured HIP graph -----------------------------------------------------
     def element_offsets(self, batch: PreparedBatch) -> np.ndarray:
@@ -262,6 +297,8 @@ class BatchPipe:
         ``GraphRun.coords`` (overwrite it in place with molecules of the same
         per-element counts) and write ``GraphRun.out``.  Per-kernel timing
         must be off while capturing."""
+        if self.add_water:
+            raise ValueError("capture: the water fill is not available in captured graphs")
         B, n_atoms = len(batch), int(batch.coords.shape[0])
         dev = device()
         mol_off = torch.as_tensor(batch.mol_off, dtype=torch.int64).to(dev)
@@ -289,11 +326,12 @@ class BatchPipe:
             launch()
         return GraphRun(graph, out, coords, (mol_off, elem_off, ws))
 
-    def exit_waves(self, batch: PreparedBatch) -> torch.Tensor:
+    def exit_waves(self, batch: PreparedBatch, seeds=None) -> torch.Tensor:
         """(B, n, n) complex64 exit waves on the GPU."""
-        return self._simulate(batch, 1)
+        return self._simulate(batch, 1, seeds=seeds)
 
-    def exit_waves_and_slices(self, batch: PreparedBatch, slices: torch.Tensor = None):
+    def exit_waves_and_slices(self, batch: PreparedBatch, slices: torch.Tensor = None,
+                              seeds=None):
         """Exit waves plus the projected potential slices the fused path built on the
         way, (B, n_slices, n, n) float32 (written into ``slices`` if given)."""
         shape = (len(batch), self.n_slices, self.n, self.n)
@@ -302,31 +340,61 @@ class BatchPipe:
         elif tuple(slices.shape) != shape or slices.dtype != torch.float32 or \
                 not slices.is_contiguous():
             raise ValueError(f"slices must be a contiguous float32 tensor of shape {shape}")
-        return self._simulate(batch, 1, slices), slices
+        return self._simulate(batch, 1, slices, seeds=seeds), slices
 
     # -- reference-shaped GPU path (cross-check) --------------------------------
-    def build_slices(self, batch: PreparedBatch) -> torch.Tensor:
+    def build_slices(self, batch: PreparedBatch, seeds=None) -> torch.Tensor:
+        """Potential slices of every molecule through the reference-shaped
+        builder calls (bin, add_water when the pipe adds water, make)."""
         nE, nS, n = len(self.elements), self.n_slices, self.n
         V = self._buffer("V", (len(batch), nS, n, n), torch.float32)
         hist = self._buffer("hist", (nE, nS, n, n), torch.float32)
         ws = self._buffer("ws", (self.builder.backend.slices_workspace_bytes(),), torch.uint8)
+        if self.add_water:
+            seeds = self.water_seeds(batch, seeds)
         for b in range(len(batch)):
             lo, hi = batch.mol_off[b], batch.mol_off[b + 1]
             self.builder.backend.bin_atoms(batch.coords[lo:hi], batch.counts[b], out=hist)
+            if self.add_water:
+                self.builder.add_water(hist, seed=int(seeds[b]))
             self.builder.backend.make_slices(hist, self.symmetric_bandlimit, out=V[b], ws=ws)
         return V
 
     # -- weak-phase-object images -------------------------------------------------
-    def run_wpo(self, batch: PreparedBatch) -> torch.Tensor:
-        """Weak-phase-object images of the whole batch (``pipe.py:57-64,73-77``):
-        each molecule's 2-D projected potential (one-slice binning,
-        ``slice_builder_base.py:202-225``), then one batched transmission,
-        the dz = 0 propagation (the 2/3 band-limit alone), lens and |psi|^2."""
+    def _one_slice_builder(self):
         from .backend.slice_builder_hip import OneSliceBuilder
-        n, nE, B = self.n, len(self.elements), len(batch)
         if getattr(self, "_one", None) is None:
-            self._one = OneSliceBuilder(self.elements, n, n, self.pixel_size, self.pixel_size)
-        one = self._one.backend
+            self._one = OneSliceBuilder(self.elements, self.n, self.n, self.pixel_size,
+                                        self.pixel_size)
+        return self._one.backend
+
+    def run_wpo(self, batch: PreparedBatch) -> torch.Tensor:
+        """Weak-phase-object images of the whole batch (``pipe.py:57-64,73-77``)
+        in one fused launch sequence: each molecule's 2-D projected potential
+        (one-slice binning, ``slice_builder_base.py:202-225``, formed from the
+        atom buckets like the multislice path's), transmission, the dz = 0
+        propagation (the 2/3 band-limit alone), lens and |psi|^2."""
+        one = self._one_slice_builder()
+        B, n_atoms = len(batch), batch.coords.shape[0]
+        out = torch.empty((B, self.n, self.n), dtype=torch.float32, device=device())
+        ws_bytes = hip_ext.raw("emsim_hip_simulate_workspace_size")(one.h, B, n_atoms, 0)
+        ws = self._buffer("wpo_ws", (max(ws_bytes, 1),), torch.uint8)
+        mic = self.microscope
+        psi0 = self.propagator.init_wave_value(mic.electron_dose)
+        hip_ext.call("emsim_hip_simulate", one.h, self.propagator.backend.h, ptr(batch.coords),
+                     n_atoms, batch.mol_off.ctypes.data_as(POINTER(c_int64)),
+                     batch.counts.ctypes.data_as(POINTER(c_int64)), B, float(psi0),
+                     float(mic.cs_mm), float(mic.defocus), float(mic.aperture),
+                     1 if self.symmetric_bandlimit else 0, ptr(out), None, 0, ptr(ws),
+                     ws.numel(), stream_handle())
+        return out
+
+    def run_wpo_unfused(self, batch: PreparedBatch) -> torch.Tensor:
+        """The same images through the reference-shaped steps, molecule by
+        molecule: one-slice histogram, make_one_slice, then one batched
+        transmission, dz = 0 propagation, lens and |psi|^2."""
+        n, nE, B = self.n, len(self.elements), len(batch)
+        one = self._one_slice_builder()
         V = self._buffer("V_wpo", (B, n, n), torch.float32)
         hist = self._buffer("hist_wpo", (nE, n, n), torch.float32)
         ws = self._buffer("ws_wpo", (max(one.slices_workspace_bytes(), 1),), torch.uint8)
@@ -340,8 +408,8 @@ class BatchPipe:
         return self.propagator.lens_intensity(waves, mic.cs_mm, mic.defocus, mic.aperture,
                                               overwrite_wave=True)
 
-    def run_unfused(self, batch: PreparedBatch) -> torch.Tensor:
-        V = self.build_slices(batch)
+    def run_unfused(self, batch: PreparedBatch, seeds=None) -> torch.Tensor:
+        V = self.build_slices(batch, seeds)
         w0 = self.propagator.init_wave(self.microscope.electron_dose, batch=len(batch))
         waves = self.propagator.multislice_propagate(w0, V, self.dz)
         mic = self.microscope

@@ -103,35 +103,6 @@ bool pow2_supported(int n) { return n >= 8 && n <= kTwN && (n & (n - 1)) == 0; }
 // --------------------------------------------------------------------------
 std::mutex g_tw_mutex;
 std::vector<float2*> g_twiddles;
-std::vector<int2*> g_twiddles_q30;
-
-// The same table in Q30 fixed point, round(2^30 cos), round(2^30 sin), for
-// the order-independent sparse DFT sums of the fused path.
-int get_twiddles_q30(const int2** out) {
-    int dev = 0;
-    EMSIM_HIP_CHECK(hipGetDevice(&dev));
-    std::lock_guard<std::mutex> lock(g_tw_mutex);
-    if ((int)g_twiddles_q30.size() <= dev) g_twiddles_q30.resize(dev + 1, nullptr);
-    if (!g_twiddles_q30[dev]) {
-        std::vector<int2> host(kTwN);
-        const double two_pi = 6.283185307179586476925286766559;
-        for (int m = 0; m < kTwN; ++m) {
-            const double a = -two_pi * (double)m / (double)kTwN;
-            host[m] = make_int2((int)std::llround(std::cos(a) * 1073741824.0),
-                                (int)std::llround(std::sin(a) * 1073741824.0));
-        }
-        int2* d = nullptr;
-        EMSIM_HIP_CHECK(hipMalloc(&d, kTwN * sizeof(int2)));
-        hipError_t err = hipMemcpy(d, host.data(), kTwN * sizeof(int2), hipMemcpyHostToDevice);
-        if (err != hipSuccess) {
-            (void)hipFree(d);
-            return fail(EMSIM_HIP_EDEVICE, "twiddle upload failed: %s", hipGetErrorString(err));
-        }
-        g_twiddles_q30[dev] = d;
-    }
-    *out = g_twiddles_q30[dev];
-    return EMSIM_HIP_OK;
-}
 
 int get_twiddles(const float2** out) {
     int dev = 0;
@@ -1119,8 +1090,7 @@ extern "C" int emsim_hip_timing_collect(char* buf, size_t cap, size_t* needed) {
 namespace {
 
 template <int N>
-int launch_spec_sparse(const uint32_t* row_start, const uint16_t* bucket, const int2* twq,
-                       const float* sf, float2* Y, int slice, int n_slices, int n_elems, int H,
+int launch_spec_sparse(const uint32_t* row_start, const uint16_t* bucket, const float* sf, float2* Y, int slice, int n_slices, int n_elems, int H,
                        int ldh, int n2, int bandlimit, int band_r2, float inv_npix, int batch,
                        const int* order, const float2* tw, hipStream_t s) {
     using Cfg = SpecCfg<N>;
@@ -1129,7 +1099,7 @@ int launch_spec_sparse(const uint32_t* row_start, const uint16_t* bucket, const 
     // with spec_pack(N), column n2/2 is carried by column 0
     const int cols = spec_pack(N) ? n2 / 2 : H;
     const dim3 grid((unsigned)(ceil_div(cols, Cfg::C) * (int64_t)batch));
-    EMSIM_LAUNCH(N, (k_spec_sparse<N>), grid, dim3(Cfg::BLOCK), 0, s, row_start, bucket, twq, sf,
+    EMSIM_LAUNCH(N, (k_spec_sparse<N>), grid, dim3(Cfg::BLOCK), 0, s, row_start, bucket, sf,
                  Y, slice, n_slices, n_elems, H, ldh, n2, batch, bandlimit, band_r2, inv_npix,
                  order, tw);
     return EMSIM_HIP_OK;
@@ -1164,11 +1134,10 @@ int launch_col_lensprop(float2* data, int ld, int ncols, int64_t img_stride, int
     return launch_col_mul_impl<N>(data, ld, ncols, img_stride, n_img, mul, tw, s, cd);
 }
 
-int spec_sparse(int n, const uint32_t* row_start, const uint16_t* bucket, const int2* twq,
-                const float* sf, float2* Y, int slice, int n_slices, int n_elems, int H, int ldh,
+int spec_sparse(int n, const uint32_t* row_start, const uint16_t* bucket, const float* sf, float2* Y, int slice, int n_slices, int n_elems, int H, int ldh,
                 int n2, int bandlimit, int band_r2, float inv_npix, int batch, const int* order,
                 const float2* tw, hipStream_t s) {
-    EMSIM_DISPATCH(n, launch_spec_sparse, row_start, bucket, twq, sf, Y, slice, n_slices,
+    EMSIM_DISPATCH(n, launch_spec_sparse, row_start, bucket, sf, Y, slice, n_slices,
                    n_elems, H, ldh, n2, bandlimit, band_r2, inv_npix, batch, order, tw, s);
 }
 int slice_pass_a(int n, bool first, const float2* Y, float2* wave, int64_t n_pairs, int ldh,
@@ -1194,12 +1163,24 @@ struct FusedWorkspace {
     uint32_t* block_sums;
     float2* Y;
     float2* wave;
+    // water fill (emsim_hip_simulate_water): dense count planes and half
+    // spectra of one slice of every molecule, per-molecule seeds
+    float* hist;
+    float2* X;
+    uint64_t* seeds;
     int64_t n_rows;
     size_t bytes;
 };
 
+// Water-fill parameters of a fused run (emsim_hip_simulate_water).
+struct WaterFill {
+    bool on = false;
+    int idx_h = 0, idx_o = 0;
+    float lam_o = 0.f, exp_o = 1.f, lam_h = 0.f, exp_h = 1.f;
+};
+
 FusedWorkspace fused_layout(const emsim_hip_slice_builder* b, int batch, int64_t n_atoms,
-                            bool own_wave, char* base) {
+                            bool own_wave, char* base, bool water = false) {
     FusedWorkspace w;
     w.n_rows = (int64_t)batch * b->n_slices * b->n_elems * b->n1;
     const int64_t n_scan = w.n_rows + 1;
@@ -1224,6 +1205,15 @@ FusedWorkspace fused_layout(const emsim_hip_slice_builder* b, int batch, int64_t
         w.wave = (float2*)(base + off);
         off += align_up(sizeof(float2) * (size_t)batch * b->n1 * b->n2);
     }
+    w.hist = nullptr;
+    w.X = nullptr;
+    w.seeds = nullptr;
+    if (water) {
+        const size_t planes = (size_t)b->n_elems * batch * b->n1;
+        w.hist = (float*)(base + off);   off += align_up(sizeof(float) * planes * b->n2);
+        w.X = (float2*)(base + off);     off += align_up(sizeof(float2) * planes * half_ld(b->H));
+        w.seeds = (uint64_t*)(base + off); off += align_up(sizeof(uint64_t) * (size_t)batch);
+    }
     w.bytes = off;
     return w;
 }
@@ -1234,8 +1224,6 @@ int check_simulate(const emsim_hip_slice_builder* b, const emsim_hip_wave_propag
     if (!b || !w || !out || batch < 1 || n_atoms < 0 || (n_atoms > 0 && !coords) ||
         (mode != 0 && mode != 1))
         return fail(EMSIM_HIP_EINVAL, "simulate: bad arguments");
-    if (b->one_slice)
-        return fail(EMSIM_HIP_EINVAL, "simulate: needs a multi-slice builder");
     if (w->n1 != b->n1 || w->n2 != b->n2)
         return fail(EMSIM_HIP_EINVAL, "simulate: builder grid %dx%d != propagator grid %dx%d",
                     b->n1, b->n2, w->n1, w->n2);
@@ -1253,7 +1241,33 @@ int check_simulate(const emsim_hip_slice_builder* b, const emsim_hip_wave_propag
 int simulate_core(const emsim_hip_slice_builder* b, const emsim_hip_wave_propagator* w,
                   const float* coords, int64_t n_atoms, const FusedWorkspace& ws, int batch,
                   float psi0, float cs_mm, float defocus, float aperture, int symmetric_bandlimit,
-                  float* out, float* slices_out, int mode, hipStream_t s);
+                  float* out, float* slices_out, int mode, hipStream_t s,
+                  const WaterFill& water = WaterFill{});
+
+// Host-side offsets of emsim_hip_simulate: elem_off (batch, n_elems + 1).
+int host_offsets(const emsim_hip_slice_builder* b, const int64_t* mol_off_host,
+                 const int64_t* elem_counts_host, int batch, int64_t n_atoms,
+                 std::vector<int64_t>& elem_off) {
+    if (mol_off_host[0] != 0 || mol_off_host[batch] != n_atoms)
+        return fail(EMSIM_HIP_EINVAL, "simulate: molecule offsets must run from 0 to n_atoms");
+    elem_off.assign((size_t)batch * (b->n_elems + 1), 0);
+    for (int m = 0; m < batch; ++m) {
+        if (mol_off_host[m + 1] < mol_off_host[m])
+            return fail(EMSIM_HIP_EINVAL, "simulate: molecule offsets must not decrease");
+        int64_t o = mol_off_host[m];
+        elem_off[(size_t)m * (b->n_elems + 1)] = o;
+        for (int e = 0; e < b->n_elems; ++e) {
+            const int64_t c = elem_counts_host[(size_t)m * b->n_elems + e];
+            if (c < 0) return fail(EMSIM_HIP_EINVAL, "simulate: negative element count");
+            o += c;
+            elem_off[(size_t)m * (b->n_elems + 1) + e + 1] = o;
+        }
+        if (o > mol_off_host[m + 1])
+            return fail(EMSIM_HIP_EINVAL, "simulate: molecule %d's element counts exceed its atoms",
+                        m);
+    }
+    return EMSIM_HIP_OK;
+}
 
 }  // namespace
 
@@ -1275,25 +1289,9 @@ extern "C" int emsim_hip_simulate(const emsim_hip_slice_builder* b,
     if (!mol_off_host || !elem_counts_host) return fail(EMSIM_HIP_EINVAL, "simulate: null offsets");
     const int check = check_simulate(b, w, coords, n_atoms, batch, symmetric_bandlimit, out, mode);
     if (check) return check;
-    // host-side offsets
-    if (mol_off_host[0] != 0 || mol_off_host[batch] != n_atoms)
-        return fail(EMSIM_HIP_EINVAL, "simulate: molecule offsets must run from 0 to n_atoms");
-    std::vector<int64_t> elem_off((size_t)batch * (b->n_elems + 1));
-    for (int m = 0; m < batch; ++m) {
-        if (mol_off_host[m + 1] < mol_off_host[m])
-            return fail(EMSIM_HIP_EINVAL, "simulate: molecule offsets must not decrease");
-        int64_t o = mol_off_host[m];
-        elem_off[(size_t)m * (b->n_elems + 1)] = o;
-        for (int e = 0; e < b->n_elems; ++e) {
-            const int64_t c = elem_counts_host[(size_t)m * b->n_elems + e];
-            if (c < 0) return fail(EMSIM_HIP_EINVAL, "simulate: negative element count");
-            o += c;
-            elem_off[(size_t)m * (b->n_elems + 1) + e + 1] = o;
-        }
-        if (o > mol_off_host[m + 1])
-            return fail(EMSIM_HIP_EINVAL, "simulate: molecule %d's element counts exceed its atoms",
-                        m);
-    }
+    std::vector<int64_t> elem_off;
+    const int rc = host_offsets(b, mol_off_host, elem_counts_host, batch, n_atoms, elem_off);
+    if (rc) return rc;
     FusedWorkspace ws = fused_layout(b, batch, n_atoms, mode == 0, (char*)workspace);
     if (!workspace || workspace_bytes < ws.bytes)
         return fail(EMSIM_HIP_EWORKSPACE, "simulate: workspace needs %zu bytes", ws.bytes);
@@ -1325,16 +1323,67 @@ extern "C" int emsim_hip_simulate_dev(const emsim_hip_slice_builder* b,
                          symmetric_bandlimit, out, slices_out, mode, (hipStream_t)stream_);
 }
 
+extern "C" size_t emsim_hip_simulate_water_workspace_size(const emsim_hip_slice_builder* b,
+                                                          int batch, int64_t n_atoms, int mode) {
+    if (!b || batch < 1 || n_atoms < 0) return 0;
+    return fused_layout(b, batch, n_atoms, mode == 0, nullptr, true).bytes;
+}
+
+extern "C" int emsim_hip_simulate_water(const emsim_hip_slice_builder* b,
+                                        const emsim_hip_wave_propagator* w, const float* coords,
+                                        int64_t n_atoms, const int64_t* mol_off_host,
+                                        const int64_t* elem_counts_host, int batch, float psi0,
+                                        float cs_mm, float defocus, float aperture,
+                                        int symmetric_bandlimit, int idx_h, int idx_o,
+                                        double voxel_water, const uint64_t* seeds_host,
+                                        float* out, float* slices_out, int mode, void* workspace,
+                                        size_t workspace_bytes, emsim_stream_t stream_) {
+    hipStream_t s = (hipStream_t)stream_;
+    if (!mol_off_host || !elem_counts_host || !seeds_host)
+        return fail(EMSIM_HIP_EINVAL, "simulate_water: null offsets or seeds");
+    int rc = check_simulate(b, w, coords, n_atoms, batch, symmetric_bandlimit, out, mode);
+    if (rc) return rc;
+    if (b->one_slice)
+        return fail(EMSIM_HIP_EINVAL, "simulate_water: needs a multi-slice builder (the "
+                    "one-slice path adds no water)");
+    if (idx_h < 0 || idx_o < 0 || idx_h >= b->n_elems || idx_o >= b->n_elems || idx_h == idx_o ||
+        !(voxel_water >= 0))
+        return fail(EMSIM_HIP_EINVAL, "simulate_water: bad water arguments");
+    if ((size_t)b->n_elems * b->n2 * sizeof(float) > 64 * 1024)
+        return fail(EMSIM_HIP_EUNSUPPORTED,
+                    "simulate_water: n_elems * n2 <= 16384 (one row of every element in LDS)");
+    std::vector<int64_t> elem_off;
+    rc = host_offsets(b, mol_off_host, elem_counts_host, batch, n_atoms, elem_off);
+    if (rc) return rc;
+    FusedWorkspace ws = fused_layout(b, batch, n_atoms, mode == 0, (char*)workspace, true);
+    if (!workspace || workspace_bytes < ws.bytes)
+        return fail(EMSIM_HIP_EWORKSPACE, "simulate_water: workspace needs %zu bytes", ws.bytes);
+    EMSIM_HIP_CHECK(hipMemcpyAsync(ws.mol_off, mol_off_host, sizeof(int64_t) * (batch + 1),
+                                   hipMemcpyHostToDevice, s));
+    EMSIM_HIP_CHECK(hipMemcpyAsync(ws.elem_off, elem_off.data(), sizeof(int64_t) * elem_off.size(),
+                                   hipMemcpyHostToDevice, s));
+    EMSIM_HIP_CHECK(hipMemcpyAsync(ws.seeds, seeds_host, sizeof(uint64_t) * batch,
+                                   hipMemcpyHostToDevice, s));
+    WaterFill wf;
+    wf.on = true;
+    wf.idx_h = idx_h;
+    wf.idx_o = idx_o;
+    // the same float32 constants as emsim_hip_add_water
+    wf.lam_o = (float)voxel_water;
+    wf.lam_h = (float)(2.0 * voxel_water);
+    wf.exp_o = (float)std::exp(-voxel_water);
+    wf.exp_h = (float)std::exp(-2.0 * voxel_water);
+    return simulate_core(b, w, coords, n_atoms, ws, batch, psi0, cs_mm, defocus, aperture,
+                         symmetric_bandlimit, out, slices_out, mode, s, wf);
+}
+
 namespace {
 
 int simulate_core(const emsim_hip_slice_builder* b, const emsim_hip_wave_propagator* w,
                   const float* coords, int64_t n_atoms, const FusedWorkspace& ws, int batch,
                   float psi0, float cs_mm, float defocus, float aperture, int symmetric_bandlimit,
-                  float* out, float* slices_out, int mode, hipStream_t s) {
-    const int2* twq = nullptr;
-    int rc = get_twiddles_q30(&twq);
-    if (rc) return rc;
-
+                  float* out, float* slices_out, int mode, hipStream_t s, const WaterFill& water) {
+    int rc = EMSIM_HIP_OK;
     BatchGeom bg;
     bg.g.e0 = b->edges;
     bg.g.e1 = b->edges + (b->n_slices + 1);
@@ -1348,7 +1397,7 @@ int simulate_core(const emsim_hip_slice_builder* b, const emsim_hip_wave_propaga
     bg.g.inv0 = b->inv_d[0];
     bg.g.inv1 = b->inv_d[1];
     bg.g.inv2 = b->inv_d[2];
-    bg.g.one_slice = 0;
+    bg.g.one_slice = b->one_slice;
     bg.g.tile_rows = 1;
     bg.g.tiles_per_plane = b->n1;
     bg.n_elems = b->n_elems;
@@ -1367,9 +1416,13 @@ int simulate_core(const emsim_hip_slice_builder* b, const emsim_hip_wave_propaga
     EMSIM_LAUNCH(0, k_scan_sums, dim3(1), dim3(kScanBlock), 0, s, ws.block_sums, n_blocks);
     EMSIM_LAUNCH(0, k_scan_add, dim3((unsigned)n_blocks), dim3(kScanBlock), 0, s, ws.row_start,
                  n_scan, ws.block_sums);
-    if (n_atoms > 0)
+    if (n_atoms > 0) {
         EMSIM_LAUNCH(0, k_csr_scatter, dim3(grid_for(n_atoms, 256)), dim3(256), 0, s, n_atoms,
                      ws.atom_row, ws.atom_y, ws.row_start, ws.row_count, ws.bucket);
+        // a fixed order of every row's atoms for the float row sums
+        EMSIM_LAUNCH(0, k_csr_sort_rows, dim3(grid_for(ws.n_rows, 256)), dim3(256), 0, s,
+                     ws.n_rows, ws.row_start, ws.bucket);
+    }
     // per-slice molecule order of the spectrum pass (identity beyond one
     // block's worth of molecules)
     const int32_t* order = nullptr;
@@ -1386,7 +1439,9 @@ int simulate_core(const emsim_hip_slice_builder* b, const emsim_hip_wave_propaga
     const float inv_npix = 1.0f / (float)((int64_t)b->n1 * b->n2);
     const int64_t npix = (int64_t)b->n1 * b->n2;
     const int64_t rows = (int64_t)batch * b->n1;
-    const PropagatorMul prop = make_propagator(w, (float)b->dz);
+    // a one-slice builder is the weak-phase-object path (emsim/pipe.py:57-64):
+    // its slice is followed by singleslice_propagate(dz = 0), the mask alone
+    const PropagatorMul prop = make_propagator(w, b->one_slice ? 0.f : (float)b->dz);
     // Columns the propagator zeroes whole are left out between the passes:
     // the wave buffer starts as zeros, pass A never stores into them and
     // pass B skips the tiles that hold only such columns (zeros from the
@@ -1402,13 +1457,34 @@ int simulate_core(const emsim_hip_slice_builder* b, const emsim_hip_wave_propaga
     }
     for (int sl = 0; sl < b->n_slices; ++sl) {
         const bool last = sl == b->n_slices - 1;
-        rc = spec_sparse(b->n1, ws.row_start, ws.bucket, twq, b->sf, ws.Y, sl, b->n_slices,
-                         b->n_elems, b->H, ldh, b->n2, symmetric_bandlimit ? 1 : 0, r * r,
-                         inv_npix, batch, order, b->tw, s);
-        if (rc) return rc;
-        rc = slice_pass_a(b->n2, sl == 0, ws.Y, wave, rows / 2, ldh, psi0, w->factor, slices_out,
-                          sl, b->n_slices, b->n1, dead, b->tw, s);
-        if (rc) return rc;
+        if (!water.on) {
+            rc = spec_sparse(b->n1, ws.row_start, ws.bucket, b->sf, ws.Y, sl, b->n_slices,
+                             b->n_elems, b->H, ldh, b->n2, symmetric_bandlimit ? 1 : 0, r * r,
+                             inv_npix, batch, order, b->tw, s);
+            if (rc) return rc;
+            rc = slice_pass_a(b->n2, sl == 0, ws.Y, wave, rows / 2, ldh, psi0, w->factor,
+                              slices_out, sl, b->n_slices, b->n1, dead, b->tw, s);
+            if (rc) return rc;
+        } else {
+            // water: dense count planes of this slice (atoms + water), then
+            // the reference-shaped R2C and spectrum column pass; Y = X[0]
+            const int ldx = half_ld(b->H);
+            const size_t lds = sizeof(float) * (size_t)b->n_elems * b->n2;
+            EMSIM_LAUNCH(0, k_dense_slice_rows, dim3((unsigned)(batch * (int64_t)b->n1)),
+                         dim3(256), lds, s, ws.row_start, ws.bucket, sl, b->n_slices,
+                         b->n_elems, b->n1, b->n2, batch, ws.hist, 1, water.idx_h, water.idx_o,
+                         water.lam_o, water.exp_o, water.lam_h, water.exp_h,
+                         (const uint64_t*)ws.seeds);
+            rc = r2c_pairs(b->n2, ws.hist, ws.X, (int64_t)b->n_elems * batch * (b->n1 / 2), b->n1,
+                           batch, batch, 0, ldx, b->tw, s);
+            if (rc) return rc;
+            rc = col_spectrum(b->n1, ws.X, b->sf, b->n_elems, batch, b->H, ldx, b->n2,
+                              symmetric_bandlimit ? 1 : 0, r * r, inv_npix, b->tw, s);
+            if (rc) return rc;
+            rc = slice_pass_a(b->n2, sl == 0, ws.X, wave, rows / 2, ldx, psi0, w->factor,
+                              slices_out, sl, b->n_slices, b->n1, dead, b->tw, s);
+            if (rc) return rc;
+        }
         if (mode == 0 && last) {
             LensPropMul lp;
             lp.p = prop;

@@ -16,8 +16,10 @@
 //      the last slice * H P / nPix, which is the lens applied to the exit
 //      wave's spectrum (F F^-1 = nPix), so the final row pass writes |psi|^2.
 //
-// R_e sums run in 64-bit fixed point (twiddles in Q30), so the result does
-// not depend on the order atoms landed in their bucket.
+// The atom scatter fills each bucket row in whatever order its atomics
+// land; k_csr_sort_rows then sorts every row's y bins, so the float32 row
+// sums R_e run over the atoms in one fixed order and results are
+// reproducible bit for bit (equal y bins add identical terms).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -28,8 +30,6 @@
 #include "kernels_wave.hpp"
 
 namespace emsim {
-
-constexpr float kQ30Inv = 9.31322574615478515625e-10f;   // 2^-30
 
 struct BatchGeom {
     BinGeom g;                 // edge tables and grid (g.n0 = nS)
@@ -62,7 +62,8 @@ __global__ void k_csr_classify(const float* __restrict__ coords, int64_t n_atoms
             int e = 0;
             while (e + 1 < bg.n_elems && a >= eo[e + 1]) ++e;
             const float z = coords[3 * a + 0], x = coords[3 * a + 1], y = coords[3 * a + 2];
-            const int s = bin_of(z, g.e0, g.n0, g.last0, g.inv0);
+            // a one-slice builder bins in 2-D (x, y) and ignores z
+            const int s = g.one_slice ? 0 : bin_of(z, g.e0, g.n0, g.last0, g.inv0);
             const int i = bin_of(x, g.e1, g.n1, g.last1, g.inv1);
             const int j = bin_of(y, g.e2, g.n2, g.last2, g.inv2);
             if (s >= 0 && i >= 0 && j >= 0) {
@@ -86,6 +87,79 @@ __global__ void k_csr_scatter(int64_t n_atoms, const int32_t* __restrict__ atom_
         if (row < 0) continue;
         const uint32_t slot = atomicSub(&row_count[row], 1u) - 1u;
         bucket[row_start[row] + slot] = atom_y[a];
+    }
+}
+
+// Sort every bucket row's y bins ascending (one thread per row; rows hold a
+// few atoms, dense ones tens, so insertion sort, and a gapped pre-pass for
+// the rare long row), which fixes the order of the float row sums.
+__global__ void k_csr_sort_rows(int64_t n_rows, const uint32_t* __restrict__ row_start,
+                                uint16_t* __restrict__ bucket) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n_rows; r += stride) {
+        const uint32_t beg = row_start[r], len = row_start[r + 1] - beg;
+        if (len < 2) continue;
+        uint16_t* a = bucket + beg;
+        // Shell-sort gaps (Ciura) for long rows; gap 1 is the insertion sort
+        const uint32_t gaps[] = {701u, 301u, 132u, 57u, 23u, 10u, 4u, 1u};
+        for (uint32_t gi = 0; gi < 8; ++gi) {
+            const uint32_t g = gaps[gi];
+            if (g >= len) continue;
+            for (uint32_t i = g; i < len; ++i) {
+                const uint16_t v = a[i];
+                uint32_t j = i;
+                while (j >= g && a[j - g] > v) {
+                    a[j] = a[j - g];
+                    j -= g;
+                }
+                a[j] = v;
+            }
+        }
+    }
+}
+
+// Dense count planes of one slice of every molecule, with the water fill.
+// One block per (molecule b, row x): the nE rows (b, slice, e, x) of the
+// atom buckets are counted into LDS (integer-valued float adds, exact and
+// order-independent), then -- with water -- every voxel y empty for all
+// elements gets Poisson oxygens and hydrogens drawn exactly as k_add_water
+// draws them for voxel v = (slice n1 + x) n2 + y of a (n_slices, n1, n2)
+// histogram with the molecule's seed; the rows are written to
+// hist (n_elems, batch, n1, n2).  Dynamic LDS: n_elems * n2 floats.
+__global__ __launch_bounds__(256) void k_dense_slice_rows(
+    const uint32_t* __restrict__ row_start, const uint16_t* __restrict__ bucket, int slice,
+    int n_slices, int n_elems, int n1, int n2, int batch, float* __restrict__ hist, int water,
+    int idx_h, int idx_o, float lam_o, float exp_o, float lam_h, float exp_h,
+    const uint64_t* __restrict__ seeds) {
+    extern __shared__ float rows[];
+    const int b = (int)(blockIdx.x / (unsigned)n1), x = (int)(blockIdx.x % (unsigned)n1);
+    const int words = n_elems * n2;
+    for (int i = threadIdx.x; i < words; i += blockDim.x) rows[i] = 0.f;
+    __syncthreads();
+    for (int e = 0; e < n_elems; ++e) {
+        const int64_t r = (((int64_t)b * n_slices + slice) * n_elems + e) * n1 + x;
+        const uint32_t beg = row_start[r], end = row_start[r + 1];
+        for (uint32_t k = beg + threadIdx.x; k < end; k += blockDim.x)
+            atomicAdd(&rows[e * n2 + bucket[k]], 1.f);
+    }
+    __syncthreads();
+    if (water) {
+        const uint64_t seed = seeds[b];
+        for (int y = threadIdx.x; y < n2; y += blockDim.x) {
+            bool empty = true;
+            for (int e = 0; e < n_elems; ++e) empty = empty && (rows[e * n2 + y] == 0.f);
+            if (!empty) continue;
+            const uint64_t v = ((uint64_t)slice * (uint64_t)n1 + (uint64_t)x) * (uint64_t)n2 +
+                               (uint64_t)y;
+            const uint64_t key = splitmix64(seed ^ splitmix64(v));
+            rows[idx_o * n2 + y] += poisson_draw(splitmix64(key + 1), exp_o, lam_o);
+            rows[idx_h * n2 + y] += poisson_draw(splitmix64(key + 2), exp_h, lam_h);
+        }
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < words; i += blockDim.x) {
+        const int e = i / n2, y = i - e * n2;
+        hist[(((int64_t)e * batch + b) * n1 + x) * n2 + y] = rows[i];
     }
 }
 
@@ -120,7 +194,7 @@ __global__ void k_slice_order(const uint32_t* __restrict__ row_start, int n_slic
 // Y rows are ldh complex apart, images n1 * ldh apart.
 // ---------------------------------------------------------------------------
 // When the (molecule, slice)'s row table (n_elems * N + 1 entries), its atom
-// y-bins and the Q30 twiddles (n2 entries) all fit, they are staged in LDS
+// y-bins and the row twiddles (n2 entries) all fit, they are staged in LDS
 // once per block.  Otherwise (dense slices) the row DFTs go through the
 // bucket in pieces: the rows t + m T of one m are adjacent, so their atoms
 // are one contiguous bucket range, staged kSpecBucketLds entries at a time;
@@ -164,39 +238,51 @@ __device__ __forceinline__ unsigned spec_item(unsigned L, unsigned total) {
 #define EMSIM_SPEC_MIN_WAVES 4
 #endif
 
-// Row DFTs of this thread's EPT rows r0 + m T at column col: the Q30
-// twiddles w^(col y) of the row's atoms summed in 64-bit integers.  rows
-// holds absolute bucket positions (minus base: the position of bkt[0]);
-// twiddle p sits at tws[p * tw_step].
+// Row DFTs of this thread's EPT rows r0 + m T at column col: the twiddles
+// w^(col y) of the row's atoms summed in float32, in bucket order (sorted).
+// rows holds absolute bucket positions (minus base: the position of
+// bkt[0]); twiddle p sits at tws[p * tw_step].
 // A packed lane (column 0 carrying column n2/2) sums the real parts of
 // both columns' twiddles: 1 into sr and (-1)^y into si.
+// col * y < 2^24 (both below 4096), so the 24-bit multiply is exact.
 template <int EPT, int T>
 __device__ __forceinline__ void row_dft(float2 (&v)[EPT], bool valid, int r0, int col, int mask,
                                         const uint32_t* __restrict__ rows, uint32_t base,
                                         const uint16_t* __restrict__ bkt,
-                                        const int2* __restrict__ tws, int tw_step, bool packed) {
+                                        const float2* __restrict__ tws, int tw_step, bool packed) {
 #pragma unroll
     for (int m = 0; m < EPT; ++m) {
-        int64_t sr = 0, si = 0;
+        float sr = 0.f, si = 0.f;
         if (valid) {
             const int r = r0 + m * T;
             const uint32_t beg = rows[r] - base, end = rows[r + 1] - base;
             if (!packed) {
-                for (uint32_t k = beg; k < end; ++k) {
-                    const int p = (col * (int)bkt[k]) & mask;
-                    const int2 w = tws[p * tw_step];
+                uint32_t k = beg;
+                // two atoms per step: both twiddle loads in flight together
+                for (; k + 2 <= end; k += 2) {
+                    const int p0 = (int)__umul24((unsigned)col, bkt[k]) & mask;
+                    const int p1 = (int)__umul24((unsigned)col, bkt[k + 1]) & mask;
+                    const float2 w0 = tws[p0 * tw_step];
+                    const float2 w1 = tws[p1 * tw_step];
+                    sr += w0.x;
+                    si += w0.y;
+                    sr += w1.x;
+                    si += w1.y;
+                }
+                if (k < end) {
+                    const float2 w = tws[((int)__umul24((unsigned)col, bkt[k]) & mask) * tw_step];
                     sr += w.x;
                     si += w.y;
                 }
             } else {
-                const int w0 = tws[0].x, wh = tws[((mask + 1) >> 1) * tw_step].x;
+                const float w0 = tws[0].x, wh = tws[((mask + 1) >> 1) * tw_step].x;
                 for (uint32_t k = beg; k < end; ++k) {
                     sr += w0;
                     si += (bkt[k] & 1) ? wh : w0;
                 }
             }
         }
-        v[m] = make_float2((float)sr * kQ30Inv, (float)si * kQ30Inv);
+        v[m] = make_float2(sr, si);
     }
 }
 
@@ -210,7 +296,7 @@ __device__ __forceinline__ void row_dft_pieces(float2 (&v)[EPT], bool valid, int
                                                int col, int mask,
                                                const uint32_t* __restrict__ rows,
                                                const uint16_t* __restrict__ bucket,
-                                               uint16_t* lds_bkt, const int2* tws, int tw_step,
+                                               uint16_t* lds_bkt, const float2* tws, int tw_step,
                                                int block, bool packed) {
 #pragma unroll
     for (int m = 0; m < EPT; ++m) {
@@ -219,7 +305,7 @@ __device__ __forceinline__ void row_dft_pieces(float2 (&v)[EPT], bool valid, int
         const uint32_t lo = __builtin_amdgcn_readfirstlane(rows[rg]);
         const uint32_t hi = __builtin_amdgcn_readfirstlane(rows[rg + T]);
         const uint32_t beg = rows[r], end = rows[r + 1];
-        int64_t sr = 0, si = 0;
+        float sr = 0.f, si = 0.f;
         for (uint32_t p0 = lo; p0 < hi; p0 += (uint32_t)kSpecBucketLds) {
             const uint32_t p1 = hi - p0 > (uint32_t)kSpecBucketLds ? p0 + kSpecBucketLds : hi;
             __syncthreads();   // the previous piece is consumed
@@ -229,13 +315,13 @@ __device__ __forceinline__ void row_dft_pieces(float2 (&v)[EPT], bool valid, int
                 const uint32_t kb = beg > p0 ? beg : p0, ke = end < p1 ? end : p1;
                 if (!packed) {
                     for (uint32_t k = kb; k < ke; ++k) {
-                        const int p = (col * (int)lds_bkt[k - p0]) & mask;
-                        const int2 w = tws[p * tw_step];
+                        const int p = (int)__umul24((unsigned)col, lds_bkt[k - p0]) & mask;
+                        const float2 w = tws[p * tw_step];
                         sr += w.x;
                         si += w.y;
                     }
                 } else {
-                    const int w0 = tws[0].x, wh = tws[((mask + 1) >> 1) * tw_step].x;
+                    const float w0 = tws[0].x, wh = tws[((mask + 1) >> 1) * tw_step].x;
                     for (uint32_t k = kb; k < ke; ++k) {
                         sr += w0;
                         si += (lds_bkt[k - p0] & 1) ? wh : w0;
@@ -243,33 +329,33 @@ __device__ __forceinline__ void row_dft_pieces(float2 (&v)[EPT], bool valid, int
                 }
             }
         }
-        v[m] = make_float2((float)sr * kQ30Inv, (float)si * kQ30Inv);
+        v[m] = make_float2(sr, si);
     }
     __syncthreads();   // lds_bkt is free again
 }
 
 // Dense slices at N >= 2048: the element's whole bucket range
 // [e_beg, e_end) is staged in pieces of CAP entries, and every thread
-// walks its EPT rows' atoms in each piece (per-row 64-bit accumulators), so
-// a piece costs one staging and two barriers however its atoms spread over
+// walks its EPT rows' atoms in each piece (per-row accumulators), so a
+// piece costs one staging and two barriers however its atoms spread over
 // the rows.  Rows come from global memory (absolute positions).
 template <int EPT, int T, int CAP, bool TWL>
 __device__ __forceinline__ void row_dft_big(float2 (&v)[EPT], bool valid, int r0, int col,
                                             int mask, const uint32_t* __restrict__ rows,
                                             uint32_t e_beg, uint32_t e_end,
                                             const uint16_t* __restrict__ bucket,
-                                            uint16_t* lds_bkt, const int2* tws, int tw_step,
+                                            uint16_t* lds_bkt, const float2* tws, int tw_step,
                                             int block, bool packed) {
-    int64_t sr[EPT], si[EPT];
+    float sr[EPT], si[EPT];
     uint32_t beg[EPT], end[EPT];
 #pragma unroll
     for (int m = 0; m < EPT; ++m) {
-        sr[m] = 0;
-        si[m] = 0;
+        sr[m] = 0.f;
+        si[m] = 0.f;
         beg[m] = rows[r0 + m * T];
         end[m] = rows[r0 + m * T + 1];
     }
-    const int w0 = tws[0].x, wh = tws[((mask + 1) >> 1) * tw_step].x;
+    const float w0 = tws[0].x, wh = tws[((mask + 1) >> 1) * tw_step].x;
     for (uint32_t p0 = e_beg; p0 < e_end; p0 += (uint32_t)CAP) {
         const uint32_t p1 = e_end - p0 > (uint32_t)CAP ? p0 + CAP : e_end;
         __syncthreads();   // the previous piece is consumed
@@ -284,15 +370,22 @@ __device__ __forceinline__ void row_dft_big(float2 (&v)[EPT], bool valid, int r0
                 // four atoms per step: their loads are in flight together
                 for (; k + 4 <= ke; k += 4) {
                     const uint16_t* yb = lds_bkt + (k - p0);
-                    const int2 a0 = tws[((col * (int)yb[0]) & mask) * tw_step];
-                    const int2 a1 = tws[((col * (int)yb[1]) & mask) * tw_step];
-                    const int2 a2 = tws[((col * (int)yb[2]) & mask) * tw_step];
-                    const int2 a3 = tws[((col * (int)yb[3]) & mask) * tw_step];
-                    sr[m] += ((int64_t)a0.x + a1.x) + ((int64_t)a2.x + a3.x);
-                    si[m] += ((int64_t)a0.y + a1.y) + ((int64_t)a2.y + a3.y);
+                    const float2 a0 = tws[((int)__umul24((unsigned)col, yb[0]) & mask) * tw_step];
+                    const float2 a1 = tws[((int)__umul24((unsigned)col, yb[1]) & mask) * tw_step];
+                    const float2 a2 = tws[((int)__umul24((unsigned)col, yb[2]) & mask) * tw_step];
+                    const float2 a3 = tws[((int)__umul24((unsigned)col, yb[3]) & mask) * tw_step];
+                    sr[m] += a0.x;
+                    si[m] += a0.y;
+                    sr[m] += a1.x;
+                    si[m] += a1.y;
+                    sr[m] += a2.x;
+                    si[m] += a2.y;
+                    sr[m] += a3.x;
+                    si[m] += a3.y;
                 }
                 for (; k < ke; ++k) {
-                    const int2 a = tws[((col * (int)lds_bkt[k - p0]) & mask) * tw_step];
+                    const float2 a =
+                        tws[((int)__umul24((unsigned)col, lds_bkt[k - p0]) & mask) * tw_step];
                     sr[m] += a.x;
                     si[m] += a.y;
                 }
@@ -306,7 +399,7 @@ __device__ __forceinline__ void row_dft_big(float2 (&v)[EPT], bool valid, int r0
     }
     __syncthreads();   // lds_bkt is free again
 #pragma unroll
-    for (int m = 0; m < EPT; ++m) v[m] = make_float2((float)sr[m] * kQ30Inv, (float)si[m] * kQ30Inv);
+    for (int m = 0; m < EPT; ++m) v[m] = make_float2(sr[m], si[m]);
 }
 
 // Column transform of the spectrum kernel, twiddles from LDS or the table.
@@ -341,7 +434,7 @@ __device__ __forceinline__ void stage_lds(T* dst, const T* __restrict__ src, uin
 template <int N>
 __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_SPEC_MIN_WAVES : 1) void k_spec_sparse(
     const uint32_t* __restrict__ row_start, const uint16_t* __restrict__ bucket,
-    const int2* __restrict__ twq, const float* __restrict__ sf, float2* __restrict__ Y,
+    const float* __restrict__ sf, float2* __restrict__ Y,
     int slice, int n_slices, int n_elems, int H, int ldh, int n2, int batch, int bandlimit,
     int band_r2, float inv_npix, const int* __restrict__ order, const float2* __restrict__ tw) {
     using Cfg = SpecCfg<N>;
@@ -350,7 +443,7 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
     __shared__ float lds_im[Cfg::C * Cfg::STRIDE];
     __shared__ uint32_t lds_rows[SpecLds<N>::kRows];
     __shared__ uint16_t lds_bucket[SpecLds<N>::kBucket];
-    __shared__ int2 lds_twq[SpecLds<N>::kTwq];
+    __shared__ float2 lds_twq[SpecLds<N>::kTwq];   // w^p of the row transforms (length n2)
     __shared__ uint32_t lds_atoms_raw[kSpecDirect > 0 ? kSpecDirect : 1];   // (x << 16) | y
     __shared__ uint32_t lds_atoms[kSpecDirect > 0 ? kSpecDirect : 1];       // the same, sorted
     // the column transforms' twiddles (N <= 512; longer columns read the table,
@@ -389,7 +482,7 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
         stage_lds<4>(lds_rows, row_start + rbase, (uint32_t)n_rows + 1, threadIdx.x, Cfg::BLOCK);
     const bool tw_lds = n2 <= SpecLds<N>::kTwq;
     if (tw_lds)
-        for (int i = threadIdx.x; i < n2; i += Cfg::BLOCK) lds_twq[i] = twq[i * tw_step];
+        for (int i = threadIdx.x; i < n2; i += Cfg::BLOCK) lds_twq[i] = tw[i * tw_step];
     if constexpr (kTwInLds) TwLds<N>::fill(lds_tw, tw, threadIdx.x, Cfg::BLOCK);
     __syncthreads();
     // block-uniform values read per lane: readfirstlane puts them in scalar
@@ -441,8 +534,8 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
         float2 v[EPT];
         if (all_lds && n_e <= (uint32_t)kSpecDirect) {
             // few atoms: their 2-D DFT sum_a w_N^(k x_a) w_n2^(col y_a) directly,
-            // cheaper than a column transform (the y factor is the Q30 twiddle
-            // the row DFT would use, the x factor the transform's table entry)
+            // cheaper than a column transform (the y factor is the twiddle the
+            // row DFT would use, the x factor the transform's table entry)
             if (c == 0) {
 #pragma unroll
                 for (int m = 0; m < EPT; ++m) {
@@ -453,9 +546,8 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
                 }
             }
             __syncthreads();
-            // sorted by (x, y), so the float sum below does not depend on the
-            // order the atoms landed in their buckets (atoms with equal keys
-            // contribute identical terms)
+            // sorted by (x, y), a fixed order for the float sum below (atoms
+            // with equal keys contribute identical terms)
             if (threadIdx.x < n_e) {
                 const uint32_t key = lds_atoms_raw[threadIdx.x];
                 uint32_t rank = 0;
@@ -471,11 +563,9 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
             for (uint32_t a = 0; a < n_e; ++a) {
                 const uint32_t xy = lds_atoms[a];
                 const int x = (int)(xy >> 16), y = (int)(xy & 0xffffu);
-                const int2 q = lds_twq[(col * y) & (n2 - 1)];
-                float2 wy = make_float2((float)q.x * kQ30Inv, (float)q.y * kQ30Inv);
+                float2 wy = lds_twq[(col * y) & (n2 - 1)];
                 if (packed)   // column 0 (1) + i column n2/2 ((-1)^y), both real
-                    wy = make_float2((float)lds_twq[0].x * kQ30Inv,
-                                     (float)((y & 1) ? lds_twq[half].x : lds_twq[0].x) * kQ30Inv);
+                    wy = make_float2(lds_twq[0].x, (y & 1) ? lds_twq[half].x : lds_twq[0].x);
 #pragma unroll
                 for (int m = 0; m < EPT; ++m) {
                     const int p = ((t + m * Cfg::T) * x) & (N - 1);
@@ -500,14 +590,14 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
                 else
                     row_dft_big<EPT, Cfg::T, SpecLds<N>::kBucket, false>(
                         v, valid, e * N + t, col, n2 - 1, row_start + rbase, e_beg, e_end, bucket,
-                        lds_bucket, twq, tw_step, Cfg::BLOCK, packed);
+                        lds_bucket, tw, tw_step, Cfg::BLOCK, packed);
             } else if (tw_lds)
                 row_dft_pieces<EPT, Cfg::T, true>(v, valid, e * N + t, t, col, n2 - 1,
                                                   row_start + rbase, bucket, lds_bucket, lds_twq,
                                                   1, Cfg::BLOCK, packed);
             else
                 row_dft_pieces<EPT, Cfg::T, false>(v, valid, e * N + t, t, col, n2 - 1,
-                                                   row_start + rbase, bucket, lds_bucket, twq,
+                                                   row_start + rbase, bucket, lds_bucket, tw,
                                                    tw_step, Cfg::BLOCK, packed);
             spec_fft<N, EPT, false, kTwInLds>(v, re, im, t, lds_tw, tw);
         }

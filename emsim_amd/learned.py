@@ -144,17 +144,24 @@ class FNO:
         hip_ext.call("emsim_hip_fno_get_grads", self.h, ptr(t), stream_handle())
         return self.unflatten(t.cpu().numpy())
 
-    def save(self, path: str) -> None:
+    def save(self, path: str, coarse_dz: float = None) -> None:
+        """Parameters and shape; coarse_dz (the slice thickness of the coarse
+        simulation the network was trained on) is stored with them."""
+        extra = {} if coarse_dz is None else {"coarse_dz": np.array([float(coarse_dz)])}
         np.savez(path, params=self.params_flat(),
-                 config=np.array([self.n, self.cin, self.C, self.modes, self.L, self.step]))
+                 config=np.array([self.n, self.cin, self.C, self.modes, self.L, self.step]),
+                 **extra)
 
     @classmethod
     def load(cls, path: str) -> "FNO":
+        """A saved network; ``coarse_dz`` is None for files saved without it
+        (networks trained on 8 A coarse slices)."""
         with np.load(path, allow_pickle=False) as z:
             n, cin, C, m, L, step = (int(v) for v in z["config"])
             f = cls(n, cin, C, m, L)
             f.set_params(z["params"])
             f.step = step
+            f.coarse_dz = float(z["coarse_dz"][0]) if "coarse_dz" in z.files else None
         return f
 
     # -- device calls --------------------------------------------------------
@@ -353,6 +360,8 @@ class LearnedPipe:
                  check_tol: float = 0.01, check_every: int = 1):
         if fno.n != pairs.n or fno.cin != 2 + pairs.coarse_slices:
             raise ValueError("network and pipelines disagree on size or slice count")
+        if pairs.fine.add_water or pairs.coarse.add_water:
+            raise ValueError("the correction network is trained on molecules without water")
         self.fno, self.pairs = fno, pairs
         self.norm_tol, self.check_tol, self.check_every = norm_tol, check_tol, check_every
         self._calls = 0

@@ -176,7 +176,11 @@ int emsim_hip_intensity(const float* wave_dev, float* out_dev, int64_t n,
  * cs_mm, defocus, aperture); mode 1: out_dev = (batch, n1, n2) complex64 exit
  * waves (the lens arguments are ignored).  slices_out_dev (nullable)
  * receives the projected potential slices, (batch, n_slices, n1, n2)
- * float32, as make_slices would return them. */
+ * float32, as make_slices would return them.
+ * With a one-slice builder this is the weak-phase-object path of
+ * emsim/pipe.py:57-64,73-77 for the whole batch: the 2-D projected potential
+ * (z ignored, slice_builder_base.py:202-225), transmission, then
+ * singleslice_propagate(dz = 0) -- the propagator's mask alone. */
 size_t emsim_hip_simulate_workspace_size(const emsim_hip_slice_builder* b, int batch,
                                          int64_t n_atoms, int mode);
 int emsim_hip_simulate(const emsim_hip_slice_builder* b, const emsim_hip_wave_propagator* w,
@@ -185,6 +189,26 @@ int emsim_hip_simulate(const emsim_hip_slice_builder* b, const emsim_hip_wave_pr
                        float defocus, float aperture, int symmetric_bandlimit, float* out_dev,
                        float* slices_out_dev, int mode, void* workspace_dev,
                        size_t workspace_bytes, emsim_stream_t stream);
+
+/* emsim_hip_simulate with the water fill of the reference's default image
+ * (Pipe(add_water=True), emsim/pipe.py:16,48-52 -> MultiSlicesBuilderBase
+ * .add_water, slice_builder_base.py:134-147): in every slice, each voxel
+ * empty for all elements gets Poisson(voxel_water) oxygens and
+ * Poisson(2 voxel_water) hydrogens in the planes of elements idx_o / idx_h.
+ * Molecule m draws with seeds_host[m] exactly the values
+ * emsim_hip_add_water draws with that seed for the molecule's own
+ * (n_slices, n1, n2) histogram.  Slices are formed from dense count planes
+ * (the reference's R2C + spectrum pass) rather than the sparse spectrum
+ * pass.  Multi-slice builders only; n_elems * n2 <= 16384. */
+size_t emsim_hip_simulate_water_workspace_size(const emsim_hip_slice_builder* b, int batch,
+                                               int64_t n_atoms, int mode);
+int emsim_hip_simulate_water(const emsim_hip_slice_builder* b, const emsim_hip_wave_propagator* w,
+                             const float* coords_dev, int64_t n_atoms, const int64_t* mol_off_host,
+                             const int64_t* elem_counts_host, int batch, float psi0, float cs_mm,
+                             float defocus, float aperture, int symmetric_bandlimit, int idx_h,
+                             int idx_o, double voxel_water, const uint64_t* seeds_host,
+                             float* out_dev, float* slices_out_dev, int mode, void* workspace_dev,
+                             size_t workspace_bytes, emsim_stream_t stream);
 
 /* ---- learned coarse-slice correction ------------------------------------
  * No counterpart in the reference (emsim has no learned component).  A
@@ -279,21 +303,6 @@ int emsim_hip_fno_force_generic(int on);
  * a training forward pass with the same x_dev and workspace. */
 int emsim_hip_fno_backward(emsim_hip_fno* f, const void* x_dev, const float* dout_dev, int batch,
                            void* workspace_dev, size_t workspace_bytes, emsim_stream_t stream);
-
-/* emsim_hip_orient_batch without a workspace argument; it allocates nothing.
- * No counterpart in the reference: it is for callers that cannot allocate
- * device memory (code under HIP graph capture, tight serving loops).  The
- * bounding-box keys live in a library-owned device array (96 KiB on each GPU
- * the library runs on, held until the process ends), so batch is at most
- * EMSIM_HIP_ORIENT_NOALLOC_MAX_BATCH (larger batches: EMSIM_HIP_EUNSUPPORTED).
- * Every call on one GPU shares that array: calls that overlap on one GPU
- * (concurrent streams, several host threads) race and must be serialised
- * by the caller.  The first call on a device looks up the array's address
- * (hipGetSymbolAddress), so make it once outside graph capture. */
-#define EMSIM_HIP_ORIENT_NOALLOC_MAX_BATCH 4096
-int emsim_hip_orient_batch_noalloc(const float* base_dev, int64_t n_atoms, const float* rot_dev,
-                                   float c0x, float c0y, float c0z, int batch, float* out_dev,
-                                   emsim_stream_t stream);
 int emsim_hip_fno_adam(emsim_hip_fno* f, float lr, float beta1, float beta2, float eps,
                        float weight_decay, int step, emsim_stream_t stream);
 /* wave_dev = coarse_wave_dev + psi0 (out_r + i out_i), complex64. */
@@ -320,6 +329,21 @@ size_t emsim_hip_orient_workspace_size(int batch);
 int emsim_hip_orient_batch(const float* base_dev, int64_t n_atoms, const float* rot_dev,
                            float c0x, float c0y, float c0z, int batch, float* out_dev,
                            void* workspace_dev, size_t workspace_bytes, emsim_stream_t stream);
+
+/* emsim_hip_orient_batch without a workspace argument; it allocates nothing.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops).  The
+ * bounding-box keys live in a library-owned device array (96 KiB on each GPU
+ * the library runs on, held until the process ends), so batch is at most
+ * EMSIM_HIP_ORIENT_NOALLOC_MAX_BATCH (larger batches: EMSIM_HIP_EUNSUPPORTED).
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (concurrent streams, several host threads) race and must be serialised
+ * by the caller.  The first call on a device looks up the array's address
+ * (hipGetSymbolAddress), so make it once outside graph capture. */
+#define EMSIM_HIP_ORIENT_NOALLOC_MAX_BATCH 4096
+int emsim_hip_orient_batch_noalloc(const float* base_dev, int64_t n_atoms, const float* rot_dev,
+                                   float c0x, float c0y, float c0z, int batch, float* out_dev,
+                                   emsim_stream_t stream);
 
 /* ---- launch-shape knobs (no counterpart in the reference; set by the
  * autotuner, emsim_amd/autotune.py) --------------------------------------
