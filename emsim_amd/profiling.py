@@ -1,13 +1,21 @@
-"""HBM traffic per kernel from rocprofv3 counter dumps.
+"""Per-kernel summaries of rocprofv3 counter dumps.
 
-rocprofv3 ``--pmc FETCH_SIZE`` / ``--pmc WRITE_SIZE`` report KiB per
-dispatch.  Each counter is scaled by what it reads for a plain copy of known
-size (``bench.py --calibrate-copy MIB``), and the result is stamped with the
-SHA-256 of the HIP library that was profiled, so a consumer can tell whether
-the numbers describe the library it is running.
+HBM traffic: rocprofv3 ``--pmc FETCH_SIZE`` / ``--pmc WRITE_SIZE`` report
+KiB per dispatch.  Each counter is scaled by what it reads for a plain copy
+of known size (``bench.py --calibrate-copy MIB``).
+
+Instruction mix: ``--pmc SQ_INSTS_VALU SQ_INSTS_LDS SQ_INSTS_SALU SQ_WAVES``
+report wave-level instruction counts per dispatch; the summary keeps their
+per-kernel means, from which a consumer derives the VALU issue fraction
+(VALU instructions / (duration x 1,024 SIMDs x 2.4 GHz / 4 cycles)).
+
+Every summary is stamped with the SHA-256 of the HIP library that was
+profiled, so a consumer can tell whether the numbers describe the library
+it is running.
 
     python -m emsim_amd.profiling FETCH.csv WRITE.csv COPY_FETCH.csv COPY_WRITE.csv \
         --copy-mib 256 --out traffic.json
+    python -m emsim_amd.profiling --sq SQ.csv --out valu.json
 """
 import argparse
 import csv
@@ -58,18 +66,42 @@ def summarize(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, copy_mib, li
             "kernels": kernels}
 
 
+SQ_COUNTERS = ("SQ_INSTS_VALU", "SQ_INSTS_LDS", "SQ_INSTS_SALU", "SQ_WAVES")
+
+# VALU issue peak of an MI355X in wave64 instructions per second: 256 CUs x 4
+# SIMDs, one wave64 VALU instruction per SIMD every 4 cycles at 2.4 GHz
+# (packed FP32 is disabled in this library, so no instruction does more).
+VALU_ISSUE_PEAK = 1024 * 2.4e9 / 4
+
+
+def summarize_sq(sq_csv, lib=None):
+    """Per-kernel mean of each SQ instruction counter per dispatch."""
+    kernels = {}
+    for counter in SQ_COUNTERS:
+        for name, (mean, count) in _per_kernel(sq_csv, counter).items():
+            rec = kernels.setdefault(name, {"dispatches": count})
+            rec[counter] = mean
+    return {"library_sha256": library_sha256(lib), "valu_issue_peak_per_s": VALU_ISSUE_PEAK,
+            "kernels": kernels}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
-    ap.add_argument("fetch_csv")
-    ap.add_argument("write_csv")
-    ap.add_argument("copy_fetch_csv")
-    ap.add_argument("copy_write_csv")
-    ap.add_argument("--copy-mib", type=int, required=True)
+    ap.add_argument("csv", nargs="*")
+    ap.add_argument("--sq", default=None, help="SQ instruction-counter CSV to summarise")
+    ap.add_argument("--copy-mib", type=int, default=0)
     ap.add_argument("--lib", default=None)
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
-    res = summarize(a.fetch_csv, a.write_csv, a.copy_fetch_csv, a.copy_write_csv, a.copy_mib,
-                    a.lib)
+    if a.sq:
+        res = summarize_sq(a.sq, a.lib)
+        Path(a.out).write_text(json.dumps(res, indent=1) + "\n")
+        print(f"wrote {a.out}: {len(res['kernels'])} kernels")
+        return
+    if len(a.csv) != 4 or a.copy_mib <= 0:
+        ap.error("traffic summary needs FETCH WRITE COPY_FETCH COPY_WRITE csv files and "
+                 "--copy-mib")
+    res = summarize(*a.csv, a.copy_mib, a.lib)
     Path(a.out).write_text(json.dumps(res, indent=1) + "\n")
     print(f"wrote {a.out}: {len(res['kernels'])} kernels, fetch x{res['fetch_scale']:.3f}, "
           f"write x{res['write_scale']:.3f}")

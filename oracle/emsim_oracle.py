@@ -188,6 +188,20 @@ def make_multi_slices_fftw32(hist, sf, symmetric_bandlimit=True):
     return np.clip(v, 1e-13, None)
 
 
+def make_multi_slices_numpy(hist, sf, symmetric_bandlimit=True):
+    """numpy backend (slice_builder_numpy.py:39-45): complex64 half spectra
+    times the float32 table, numpy's sum over elements, normalised irfft2,
+    band-limit (float64 under numpy 2.x), clip."""
+    n_e, n_s, n1, n2 = hist.shape
+    phase = np.fft.rfft2(hist.astype(F32), axes=(-2, -1))        # complex64
+    phase *= sf[:, None, :, :]
+    slices = np.fft.irfft2(np.sum(phase, axis=0), s=(n1, n2))
+    if symmetric_bandlimit:
+        slices = symmetric_bandlimit_real(slices)
+    np.clip(slices, 1e-13, None, out=slices)
+    return slices
+
+
 def make_multi_slices_f64(hist, sf, symmetric_bandlimit=True):
     """FP64 oracle of the same potential build."""
     n_e, n_s, n1, n2 = hist.shape

@@ -105,3 +105,18 @@ def test_one_slice_equals_multi_slice_with_one_plane():
     a = orc.make_one_slice_fftw32(hist, sf)
     b = orc.make_multi_slices_fftw32(hist[:, None], sf)[0]
     assert orc.rel_l2(a, b) < 2e-6
+
+
+@pytest.mark.parametrize("n,bandlimit", [(8, True), (16, False)])
+def test_numpy_backend_slices_against_explicit_dft(n, bandlimit):
+    """The numpy backend's multi-slice build (slice_builder_numpy.py:39-45):
+    same potential as the explicit-DFT brute force (float32 spectra, so
+    1e-5), and for one slice exactly the one-slice builder's operations."""
+    rng = np.random.default_rng(100 + n)
+    hist = rng.integers(0, 3, size=(len(ZS), 4, n, n)).astype(np.float32)
+    sf = orc.scattering_factors_half(ZS, n, n, 0.5, 0.5)
+    got = orc.make_multi_slices_numpy(hist, sf, bandlimit)
+    ref = brute_slices(hist.astype(np.float64), sf.astype(np.float64), bandlimit)
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-5
+    one = orc.make_one_slice_fftw32(hist[:, 1], sf, bandlimit)
+    assert np.array_equal(orc.make_multi_slices_numpy(hist[:, 1:2], sf, bandlimit)[0], one)
