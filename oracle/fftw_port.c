@@ -110,10 +110,12 @@ static int bin_index(float x, const double* e, int nb) {
 }
 
 /* ---- the whole per-image path ------------------------------------------ */
-int port_simulate(const float* coords, const int64_t* counts, int n_elems, int n_slices, int n,
-                  double dz, double d, const float* sf /* (nE, n, n/2+1) */, float lam,
-                  float gamma, float cs_mm, float defocus, float aperture, float psi0,
-                  int bandlimit, float* image /* (n, n) */) {
+/* exit_wave (nullable): (n, n) complex64 as interleaved floats, the wave
+ * after the last slice (before the lens). */
+int port_simulate_ex(const float* coords, const int64_t* counts, int n_elems, int n_slices, int n,
+                     double dz, double d, const float* sf /* (nE, n, n/2+1) */, float lam,
+                     float gamma, float cs_mm, float defocus, float aperture, float psi0,
+                     int bandlimit, float* image /* (n, n) */, float* exit_wave) {
     const int H = n / 2 + 1;
     const size_t npix = (size_t)n * n;
     double* e0 = (double*)malloc(sizeof(double) * (n_slices + 1));
@@ -151,13 +153,32 @@ int port_simulate(const float* coords, const int64_t* counts, int n_elems, int n
         a_xy[a] = i * n + j;
     }
 
+    /* atoms grouped by slice (counting sort, atom order kept within a slice) */
+    int64_t* s_off = (int64_t*)calloc((size_t)n_slices + 1, sizeof(int64_t));
+    int64_t* s_atoms = (int64_t*)malloc(sizeof(int64_t) * (n_atoms ? n_atoms : 1));
+    int* a_e = (int*)malloc(sizeof(int) * (n_atoms ? n_atoms : 1));
+    {
+        int64_t a = 0;
+        for (int e = 0; e < n_elems; ++e)
+            for (int64_t k = 0; k < counts[e]; ++k, ++a) {
+                a_e[a] = e;
+                if (a_s[a] >= 0) ++s_off[a_s[a] + 1];
+            }
+        for (int s = 0; s < n_slices; ++s) s_off[s + 1] += s_off[s];
+        int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * ((size_t)n_slices + 1));
+        memcpy(fill, s_off, sizeof(int64_t) * ((size_t)n_slices + 1));
+        for (a = 0; a < n_atoms; ++a)
+            if (a_s[a] >= 0) s_atoms[fill[a_s[a]]++] = a;
+        free(fill);
+    }
+
     for (int s = 0; s < n_slices; ++s) {
         /* histogram planes of this slice */
         memset(hist, 0, sizeof(float) * (size_t)n_elems * npix);
-        int64_t a = 0;
-        for (int e = 0; e < n_elems; ++e)
-            for (int64_t k = 0; k < counts[e]; ++k, ++a)
-                if (a_s[a] == s) hist[(size_t)e * npix + a_xy[a]] += 1.0f;
+        for (int64_t k = s_off[s]; k < s_off[s + 1]; ++k) {
+            const int64_t a = s_atoms[k];
+            hist[(size_t)a_e[a] * npix + a_xy[a]] += 1.0f;
+        }
         /* spectrum: sum_e F_e FFT2(h_e) (float32, element order) */
         memset(spec, 0, sizeof(cf) * npix);
         for (int e = 0; e < n_elems; ++e) {
@@ -216,6 +237,7 @@ int port_simulate(const float* coords, const int64_t* counts, int n_elems, int n
             }
         fft2(wave, n, tw, 1);
     }
+    if (exit_wave) memcpy(exit_wave, wave, sizeof(cf) * npix);
     /* lens and |psi|^2 */
     {
         float c1 = 0.5f * PI32 * cs_mm * 1e7f * lam * lam * lam;
@@ -243,9 +265,16 @@ int port_simulate(const float* coords, const int64_t* counts, int n_elems, int n
         fft2(wave, n, tw, 1);
         for (size_t i = 0; i < npix; ++i) image[i] = wave[i].re * wave[i].re + wave[i].im * wave[i].im;
     }
-    free(a_s); free(a_xy); free(e0); free(e1); free(tw);
+    free(a_s); free(a_xy); free(a_e); free(s_off); free(s_atoms); free(e0); free(e1); free(tw);
     free(hist); free(spec); free(plane); free(wave); free(V);
     return 0;
+}
+
+int port_simulate(const float* coords, const int64_t* counts, int n_elems, int n_slices, int n,
+                  double dz, double d, const float* sf, float lam, float gamma, float cs_mm,
+                  float defocus, float aperture, float psi0, int bandlimit, float* image) {
+    return port_simulate_ex(coords, counts, n_elems, n_slices, n, dz, d, sf, lam, gamma, cs_mm,
+                            defocus, aperture, psi0, bandlimit, image, NULL);
 }
 
 int port_threads(void) {

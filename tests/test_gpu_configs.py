@@ -89,3 +89,34 @@ def test_mixed_sizes_batch_invariant_and_match_oracle():
         assert err < TOL
         checked += 1
     assert checked == 2
+
+
+def test_large_slab_full_400_slices_matches_oracle_fixture():
+    """The large slab at its full depth: 2048^2, 0.5 A pixels, all 400 slices of
+    1 A through 42.3M atoms (water plus five 50k-atom clouds), against the CPU
+    oracle's result at the same configuration (tests/golden/make_c3_fixture.py
+    ran oracle/fftw_port.c on the same synthetic atoms): image and exit-wave
+    crops around the centre, 32 x 32 block sums of the whole image, and the
+    image's total and sum of squares.  This is where float32 rounding
+    accumulates over the most slices."""
+    from pathlib import Path
+    from tests.golden import make_c3_fixture as fx_mod
+    fx = np.load(Path(fx_mod.__file__).with_name("c3_slab_2048x400.npz"))
+    tem = _tem()
+    mol = synth.large_slab(seed=1)
+    pipe = BatchPipe(tem, fx_mod.PIXEL, fx_mod.DZ, fx_mod.N, fx_mod.NS, ELEMS)
+    prep = pipe.prepare(mol)
+    assert len(mol) == int(fx["n_atoms"][0])
+    assert np.array_equal(prep.counts, fx["counts"])
+    print(f"[large slab 400] {len(mol)} atoms, {fx_mod.N}^2 x {fx_mod.NS} slices", flush=True)
+    batch = pipe.upload([prep])
+    del mol, prep
+    wave = pipe.exit_waves(batch)[0].cpu().numpy()
+    image = pipe.run(batch)[0].cpu().numpy()
+    got = fx_mod.summarize(image, wave)
+    errs = {k: orc.rel_l2(got[k], fx[k]) for k in ("image_crop", "wave_crop", "block_sums")}
+    for k in ("image_sum", "image_sumsq"):
+        errs[k] = abs(float(got[k][0]) - float(fx[k][0])) / abs(float(fx[k][0]))
+    print("[large slab 400] rel. differences vs oracle: " +
+          ", ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
+    assert all(v < TOL for v in errs.values()), errs

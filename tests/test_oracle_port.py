@@ -26,3 +26,7 @@ def test_port_matches_numpy_oracle(n, ns, dz, pixel):
     got = port.simulate(mol.coordinates, counts, ns, n, dz, pixel, sf, lam, gamma, 1.3, 700.0,
                         10e-3, float(w0[0, 0].real))
     assert orc.rel_l2(got, ref) < 1e-5
+    img, wave = port.simulate(mol.coordinates, counts, ns, n, dz, pixel, sf, lam, gamma, 1.3,
+                              700.0, 10e-3, float(w0[0, 0].real), exit_wave=True)
+    assert np.array_equal(img, got)
+    assert wave.dtype == np.complex64 and orc.rel_l2(wave, x) < 1e-5
