@@ -91,8 +91,12 @@ def tune_batch(make_run: Callable[[int], Callable[[], object]], widths: Iterable
 
 
 def default_cache_path() -> Path:
-    return Path(os.environ.get("EMSIM_TUNING_CACHE",
-                               Path.home() / ".cache" / "emsim_amd" / "tuning.json"))
+    """EMSIM_TUNING_CACHE, else ``tuning_cache.json`` beside the built HIP
+    library (inside the repository, so results travel with the build)."""
+    env = os.environ.get("EMSIM_TUNING_CACHE")
+    if env:
+        return Path(env)
+    return Path(__file__).resolve().parent / "backend" / "hip_ext" / "tuning_cache.json"
 
 
 def cached(key: str, run: Callable[[], object], path: Optional[Path] = None, **kw) -> Dict[str, int]:

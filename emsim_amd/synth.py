@@ -140,8 +140,10 @@ def mixed_workload(count: int = 256, seed: int = 2, lo: int = 5000, hi: int = 20
     n_atoms = np.exp(rng.uniform(np.log(lo), np.log(hi), count)).astype(int)
     wave = rng.choice(np.asarray(sizes), size=count)
     quats = random_uniform_quaternions(count, rng=np.random.default_rng(seed))
-    return [(rotated(protein_cloud(int(n), seed + 100 + i), quats[i]), int(w))
-            for i, (n, w) in enumerate(zip(n_atoms, wave))]
+    from .batch import pool_map   # clouds are independent: build them on a thread pool
+    mols = pool_map(lambda i: rotated(protein_cloud(int(n_atoms[i]), seed + 100 + i), quats[i]),
+                    range(count))
+    return [(m, int(w)) for m, w in zip(mols, wave)]
 
 
 def five_atoms() -> AtomList:

@@ -1,14 +1,15 @@
 """Train the learned coarse-slice correction on pairs the plain GPU path makes.
 
-    python -m emsim_amd.train_fno --steps 2500 --batch 8 --out fno_512.npz
+    python -m emsim_amd.train_fno --steps 2500 --batch 8 --coarse-dz 16 --out fno_512.npz
 
-Every step simulates a fresh batch of synthetic protein-like clouds (10k-100k
-atoms, log-uniform, training seeds 100 and up, random orientations) twice
-with the plain fused multislice kernels -- at 8 A and at 1 A slices -- and
-takes one Adam step of the network on (coarse exit wave, coarse potentials)
--> fine exit wave.  At the end the network is evaluated on held-out clouds
-(seeds 10000 and up, the benchmark's) against plain 1 A multislice, with and
-without the correction.
+Every step draws a batch of synthetic protein-like clouds (atom counts
+log-uniform in 10k-100k) from a pool of base clouds (seeds 101 and up) in
+fresh random orientations, simulates them twice with the plain fused
+multislice kernels -- at the coarse slice thickness and at 1 A -- and takes
+one Adam step of the network on (coarse exit wave, coarse potentials) ->
+fine exit wave.  At the end the network is evaluated on held-out clouds
+(seeds 10001 and up, disjoint from the pool and from the benchmark's
+20001-20064) against plain 1 A multislice, with and without the correction.
 """
 import argparse
 import json
@@ -28,10 +29,12 @@ def default_tem() -> TEM:
 
 
 def cloud_pool(count: int, seed: int, lo: int = 10000, hi: int = 100000):
-    """count base clouds, atom counts log-uniform in [lo, hi]."""
+    """count base clouds, atom counts log-uniform in [lo, hi], cloud seeds
+    seed + 1 ... seed + count (built on a thread pool)."""
+    from .batch import pool_map
     rng = np.random.default_rng(seed)
     sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), count)).astype(int)
-    return [synth.protein_cloud(int(s), seed + 1 + i) for i, s in enumerate(sizes)]
+    return pool_map(lambda i: synth.protein_cloud(int(sizes[i]), seed + 1 + i), range(count))
 
 
 def batch_source(pipe, pool, batch: int, seed: int):
@@ -40,7 +43,9 @@ def batch_source(pipe, pool, batch: int, seed: int):
         rng = np.random.default_rng([seed, step])
         idx = rng.integers(0, len(pool), batch)
         quats = random_uniform_quaternions(batch, rng=rng)
-        return [pipe.prepare(synth.rotated(pool[i], q)) for i, q in zip(idx, quats)]
+        from .batch import pool_map
+        return pool_map(lambda k: pipe.prepare(synth.rotated(pool[idx[k]], quats[k])),
+                        range(batch))
     return batches
 
 
@@ -64,13 +69,13 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--slices", type=int, default=100)
-    ap.add_argument("--coarse-dz", type=float, default=8.0)
+    ap.add_argument("--coarse-dz", type=float, default=16.0)
     ap.add_argument("--channels", type=int, default=32)
     ap.add_argument("--modes", type=int, default=16)
     ap.add_argument("--layers", type=int, default=4)
     ap.add_argument("--lr", type=float, default=2e-3)
-    ap.add_argument("--pool", type=int, default=24, help="base training clouds")
-    ap.add_argument("--eval", type=int, default=16, help="held-out evaluation images")
+    ap.add_argument("--pool", type=int, default=256, help="base training clouds (seeds 101+)")
+    ap.add_argument("--eval", type=int, default=32, help="held-out evaluation images")
     ap.add_argument("--init", default=None, help="checkpoint to continue from")
     ap.add_argument("--out", required=True)
     ap.add_argument("--log", default=None, help="JSON history file")
@@ -80,6 +85,8 @@ def main(argv=None):
     coarse_slices = int(np.ceil(a.slices * 1.0 / a.coarse_dz))
     pairs = CoarseFinePairs(tem, n=a.n, fine_slices=a.slices, coarse_dz=a.coarse_dz,
                             coarse_slices=coarse_slices)
+    if a.pool > 9000:
+        ap.error("--pool must stay below 9000 (seeds 10001+ are the held-out clouds)")
     if a.init:
         fno = FNO.load(a.init)
     else:
@@ -91,10 +98,10 @@ def main(argv=None):
     hist = train(fno, pairs, batch_source(pairs.fine, pool, a.batch, seed=100), a.steps,
                  lr=a.lr, log_every=max(a.steps // 25, 1),
                  log=lambda s: print(s, flush=True))
-    fno.save(a.out)
-    eval_pool = cloud_pool(8, seed=10000)
+    fno.save(a.out, coarse_dz=a.coarse_dz)
+    eval_pool = cloud_pool(a.eval, seed=10000)     # one distinct cloud per image
     quats = random_uniform_quaternions(a.eval, rng=np.random.default_rng(2))
-    mols = [synth.rotated(eval_pool[i % len(eval_pool)], quats[i]) for i in range(a.eval)]
+    mols = [synth.rotated(eval_pool[i], quats[i]) for i in range(a.eval)]
     ev = evaluate(fno, pairs, mols)
     print("held-out:", json.dumps(ev), flush=True)
     if a.log:

@@ -4,12 +4,11 @@ Restates ``emsim/utils/rot.py`` (reference): Shoemake's uniform quaternion
 draw (``:8-36``, three ``np.random.random`` numbers per quaternion, so a
 seeded global RNG reproduces the reference's draws), the batched draw
 (``:39-53``), the axis-angle quaternion (``:82-104``) and the matrix
-conversion (``:107-157``).  The matrix keeps the reference's convention,
+conversion (``:107-157``).  The reference's axis-angle matrix helper
+(``:56-79``) has no caller on the hot path and is not restated.  The matrix keeps the reference's convention,
 which is the transpose of the usual active rotation matrix
 (``rot[0, 1] = 2 (q1 q2 + q0 q3)``).
 """
-import math
-
 import numpy as np
 
 
@@ -22,21 +21,16 @@ def random_uniform_quaternion(rng=None) -> np.ndarray:
 
 
 def random_uniform_quaternions(n: int, rng=None) -> np.ndarray:
-    """(n, 4) quaternions, drawn one after another as the reference does."""
-    return np.array([random_uniform_quaternion(rng) for _ in range(n)])
+    """(n, 4) quaternions, drawn one after another as the reference does.
 
-
-def rotation_matrix(axis, theta: float) -> np.ndarray:
-    """Counter-clockwise rotation about ``axis`` by ``theta`` (``rot.py:56-79``)."""
-    axis = np.asarray(axis, dtype=np.float64)
-    axis = axis / math.sqrt(np.dot(axis, axis))
-    a = math.cos(theta / 2.0)
-    b, c, d = -axis * math.sin(theta / 2.0)
-    aa, bb, cc, dd = a * a, b * b, c * c, d * d
-    bc, ad, ac, ab, bd, cd = b * c, a * d, a * c, a * b, b * d, c * d
-    return np.array([[aa + bb - cc - dd, 2 * (bc + ad), 2 * (bd - ac)],
-                     [2 * (bc - ad), aa + cc - bb - dd, 2 * (cd + ab)],
-                     [2 * (bd + ac), 2 * (cd - ab), aa + dd - bb - cc]])
+    The n draws of three uniforms are taken in one call: numpy fills an
+    array of uniforms from the same stream, in the same order, as n calls of
+    three, so the quaternions equal the one-at-a-time ones bit for bit."""
+    u = np.random.random((n, 3)) if rng is None else rng.random((n, 3))
+    s1, s2 = np.sqrt(1 - u[:, 0]), np.sqrt(u[:, 0])
+    t2, t3 = 2 * np.pi * u[:, 1], 2 * np.pi * u[:, 2]
+    return np.stack([s1 * np.sin(t2), s1 * np.cos(t2), s2 * np.sin(t3), s2 * np.cos(t3)],
+                    axis=1).reshape(n, 4)
 
 
 def get_quaternion(n, theta: float) -> np.ndarray:

@@ -157,3 +157,22 @@ def test_pool_map_keeps_order_and_raises():
         return x
     with pytest.raises(ValueError):
         pool_map(bad, range(10), workers=4)
+
+
+def test_batched_quaternion_draws_equal_one_at_a_time():
+    """random_uniform_quaternions draws all uniforms in one call; the values,
+    and the generator state after the draw, equal n one-at-a-time Shoemake
+    draws (the reference's order, rot.py:8-53), for a Generator and for
+    numpy's global RNG."""
+    from emsim_amd.utils.rot import random_uniform_quaternion, random_uniform_quaternions
+    for seed in (0, 3, 20000):
+        r1, r2 = np.random.default_rng(seed), np.random.default_rng(seed)
+        one = np.array([random_uniform_quaternion(r1) for _ in range(257)])
+        assert np.array_equal(one, random_uniform_quaternions(257, rng=r2))
+        assert r1.random() == r2.random()
+        np.random.seed(seed)
+        one = np.array([random_uniform_quaternion() for _ in range(9)])
+        np.random.seed(seed)
+        assert np.array_equal(one, random_uniform_quaternions(9))
+    q = random_uniform_quaternions(1000, rng=np.random.default_rng(1))
+    assert np.allclose(np.linalg.norm(q, axis=1), 1.0)
