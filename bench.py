@@ -496,6 +496,11 @@ def cpu_rows(tem, mols):
 
 
 # ---------------------------------------------------------------------------
+def progress(msg: str) -> None:
+    """A progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def calibrate_copy(mib: int):
     import torch
     n = mib * (1 << 20) // 4
@@ -531,6 +536,7 @@ def main():
     tem = emsim_amd.tem.TEM(electron_dose=20, beam_energy_kev=200, cs_mm=1.3, defocus=700,
                             aperture=10e-3)
     mols, sizes = workload(args.batch)
+    progress(f"workload: {args.batch} molecules, mean {np.mean(sizes):.0f} atoms")
     if args.learned_only:
         print(json.dumps(learned_bench(args, cfg, tem, mols, None)), flush=True)
         return
@@ -567,6 +573,7 @@ def main():
     n_images = args.batch * args.steps
     value = n_images / elapsed
     ms_per_step = elapsed / args.steps * 1e3
+    progress(f"plain path: {value:.1f} images/s ({ms_per_step:.2f} ms per batch)")
 
     # atoms inside the simulated box (binned), for the spectrum pass's bytes
     half = np.array([cfg["slices"] * cfg["dz"], cfg["n"] * cfg["pixel"],
@@ -644,17 +651,21 @@ def main():
         del images
         torch.cuda.empty_cache()
         learned = learned_bench(args, cfg, tem, mols, value)
+        progress(f"learned line: {learned['images_per_s']} images/s")
 
     e2e = None
     if not args.no_e2e and not args.unfused:
         e2e = end_to_end(pipe, mols, args.batch)
+        progress(f"host-to-host line: {e2e['images_per_s']} images/s")
 
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(mols, cfg, tem)
+        progress(f"CPU baseline: {cpu['value']:.3f} images/s")
     rows = None
     if not args.no_cpu_rows and args.n == 512 and args.slices == 100:
         rows = cpu_rows(tem, mols)
+        progress("CPU rows done")
 
     line = {
         "metric": "images/s at 512^2 x 100 slices (plain multislice, full path atoms->image)",
