@@ -51,6 +51,8 @@ def parse():
                     help="skip the CPU restatements of the other configurations")
     ap.add_argument("--no-e2e", action="store_true",
                     help="skip the host-atoms -> host-images (BatchSim) line")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the other configurations (C2 orientations, C3 slab, C4 mixed)")
     ap.add_argument("--unfused", action="store_true",
                     help="time the reference-shaped kernels (histogram, make_slices, multislice)")
     ap.add_argument("--no-timing", action="store_true",
@@ -330,6 +332,92 @@ def end_to_end(pipe, mols, batch: int, repeats: int = 2):
             "region": "host AtomLists (unsorted, uncentred) -> host float32 images: "
                       "host preparation on a thread pool, upload, fused GPU path, "
                       "pinned D2H ring overlapped with the next batch"}
+
+
+# ---------------------------------------------------------------------------
+# other configurations on the GPU
+# ---------------------------------------------------------------------------
+def _kernel_ms(fn):
+    """Summed kernel time (ms) of one call of fn, from per-launch events."""
+    import torch
+    from emsim_amd.backend import hip_ext
+    hip_ext.timing_collect()
+    hip_ext.timing_enable(True)
+    fn()
+    torch.cuda.synchronize()
+    hip_ext.timing_enable(False)
+    return sum(k["total_ms"] for k in hip_ext.timing_collect())
+
+
+def _wall(fn, reps):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def config_lines(tem):
+    """C2 (one 50k-atom cloud in 64 orientations rotated on the device), C3
+    (the 2048^2 x 400-slice slab, one image), C4 (256 molecules of 5k-200k
+    atoms on 128^2/256^2/512^2 waves, from host AtomLists as they come):
+    images/s and the fraction of the wall time the GPU spends in kernels."""
+    from emsim_amd import atoms as atm
+    from emsim_amd import synth
+    from emsim_amd.batch import BatchPipe, MixedBatch
+    from emsim_amd.utils.rot import random_uniform_quaternions
+    out = {}
+    # C2
+    mol = synth.protein_cloud(50000, 0)
+    ns2 = int(np.ceil(float(atm.centralize(mol).space[0]) / 2.0))
+    p2 = BatchPipe(tem, 1.0, 2.0, 256, ns2, [1, 6, 7, 8, 16])
+    b2 = p2.upload_orientations(mol, random_uniform_quaternions(64, rng=np.random.default_rng(2)))
+    t = _wall(lambda: p2.run(b2), 5)
+    k = _kernel_ms(lambda: p2.run(b2))
+    one = p2.upload([p2.prepare(mol)])
+    t1 = _wall(lambda: p2.run(one), 5)
+    out["C2"] = {"workload": f"50k-atom cloud (seed 0) in 64 orientations, 256^2, {ns2} slices "
+                             f"of 2 A, 1 A pixels",
+                 "images_per_s": round(64 / t, 1), "ms_per_image_batch1": round(t1 * 1e3, 3),
+                 "gpu_busy_frac": round(k * 1e-3 / t, 3)}
+    progress(f"C2: {out['C2']['images_per_s']} images/s")
+    del b2, one, p2
+    # C4
+    t0 = time.perf_counter()
+    items = synth.mixed_workload(count=256, seed=2, lo=5000, hi=200000)
+    gen = time.perf_counter() - t0
+    mb = MixedBatch(tem, 1.0, 2.0, [1, 6, 7, 8, 16])
+    t = _wall(lambda: mb.run(items), 2)
+    k = _kernel_ms(lambda: mb.run(items))
+    th = _wall(lambda: mb.run(items, device_prep=False), 1)
+    sizes = np.array([len(m) for m, _ in items])
+    out["C4"] = {"workload": f"256 molecules of {sizes.min()}-{sizes.max()} atoms (mean "
+                             f"{sizes.mean():.0f}), waves 128^2/256^2/512^2, 1 A pixels, 2 A "
+                             f"slices; host AtomLists -> GPU images, preparation on the device",
+                 "images_per_s": round(256 / t, 1), "ms_per_256": round(t * 1e3, 1),
+                 "gpu_busy_frac": round(k * 1e-3 / t, 3),
+                 "groups": [(n, ns, len(idx)) for n, ns, idx in mb.last_groups],
+                 "host_prepared_images_per_s": round(256 / th, 1),
+                 "generation_s": round(gen, 1)}
+    progress(f"C4: {out['C4']['images_per_s']} images/s")
+    del items, mb
+    # C3
+    slab = synth.large_slab(seed=1)
+    p3 = BatchPipe(tem, 0.5, 1.0, 2048, 400, [1, 6, 7, 8, 16])
+    b3 = p3.upload([p3.prepare(slab)])
+    n_atoms = len(slab)
+    del slab
+    t = _wall(lambda: p3.run(b3), 2)
+    k = _kernel_ms(lambda: p3.run(b3))
+    out["C3"] = {"workload": f"2048^2 x 400 slices of 1 A, 0.5 A pixels, {n_atoms} atoms "
+                             f"(water slab + five 50k-atom clouds), one image",
+                 "images_per_s": round(1 / t, 3), "ms_per_image": round(t * 1e3, 1),
+                 "gpu_busy_frac": round(k * 1e-3 / t, 3)}
+    progress(f"C3: {out['C3']['ms_per_image']} ms per image")
+    return out
 
 
 # ---------------------------------------------------------------------------
@@ -658,6 +746,12 @@ def main():
         e2e = end_to_end(pipe, mols, args.batch)
         progress(f"host-to-host line: {e2e['images_per_s']} images/s")
 
+    configs = None
+    if not args.no_configs and not args.unfused:
+        torch.cuda.empty_cache()
+        configs = config_lines(tem)
+        torch.cuda.empty_cache()
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(mols, cfg, tem)
@@ -695,6 +789,7 @@ def main():
         "kernels": ktable,
         "learned": learned,
         "end_to_end": e2e,
+        "configs": configs,
         "cpu_baseline": cpu,
         "cpu_rows": rows,
     }

@@ -2,17 +2,19 @@
 
 The library's launch-shape knobs (``emsim_hip_tuning_set``) decide how the
 work of one pass is split over thread blocks -- rows (row pairs) per
-row-worker block and column tiles per column-kernel block -- and never the
-arithmetic, so every setting gives bit-identical images.  ``tune`` times a
+row-worker block, column tiles per column-kernel block, and the propagator
+pass's column-transform tile (values per thread and columns per block) --
+and never the arithmetic (each butterfly computes the same values whichever
+thread runs it), so every setting gives bit-identical images.  ``tune`` times a
 workload under candidate settings by coordinate descent (one knob at a time,
 the others held at their best so far) and returns the fastest; ``cached``
 keeps results per (device, workload) in a JSON file so a serving process
 tunes once.
 
-The FFT radix plan and the tile widths are compile-time properties of the
-kernels (one leading radix-2/4/8 pass then radix 8; 8 values per thread);
-the batch width is the caller's choice and can be tuned the same way with
-``tune_batch``.
+The FFT radix plan is fixed by the length (one leading radix-2/4/8 pass then
+radix-8 passes); the column-transform tiles of each length are compiled in
+(``col_shape`` picks one at run time); the batch width is the caller's
+choice and can be tuned the same way with ``tune_batch``.
 """
 import json
 import os
@@ -23,6 +25,10 @@ from typing import Callable, Dict, Iterable, Optional, Sequence, Tuple
 KNOBS: Dict[str, Tuple[int, ...]] = {
     "row_items_per_block": (0, 1, 2, 4, 8, 16),
     "col_tiles_per_block": (0, 1, 2, 4, 8),
+    # column-transform tile of the propagator pass (256, 512, 1024 points):
+    # 0 = 8 values per thread, 8 columns per block; 1 = half the columns per
+    # block; 2 = 16 values per thread (two radix-8 butterflies per pass)
+    "col_shape": (0, 1, 2),
 }
 
 

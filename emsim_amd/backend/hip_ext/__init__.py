@@ -82,6 +82,11 @@ _SIGNATURES = {
                                          c_float, c_int, c_int, c_int, c_double,
                                          POINTER(c_uint64), c_void_p, c_void_p, c_int, c_void_p,
                                          c_size_t, c_void_p]),
+    "emsim_hip_simulate_raw_workspace_size": (c_size_t, [c_void_p, c_int, c_int64, c_int]),
+    "emsim_hip_simulate_raw": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64,
+                                       POINTER(c_int64), c_int, c_float, c_float, c_float,
+                                       c_float, c_int, c_void_p, c_void_p, c_int, c_void_p,
+                                       c_size_t, c_void_p]),
     "emsim_hip_gemm": (c_int, [c_void_p, c_void_p]),
     "emsim_hip_fno_create": (c_int, [POINTER(c_void_p), c_int, c_int, c_int, c_int, c_int]),
     "emsim_hip_fno_destroy": (None, [c_void_p]),

@@ -65,6 +65,30 @@ class PreparedBatch:
         return len(self.labels)
 
 
+@dataclass
+class RawBatch:
+    """Molecules as they come (not centred, atoms in any order) on the GPU,
+    for the device-side preparation of ``BatchPipe.run_raw``."""
+    coords: torch.Tensor      # (total atoms, 3) float32 on the GPU
+    elem: torch.Tensor        # (total atoms,) uint8: element index in the pipe's list
+    mol_off: np.ndarray       # (B+1,) int64
+    labels: List[str]
+
+    def __len__(self):
+        return len(self.labels)
+
+
+def centred_thickness(mol: atm.AtomList) -> float:
+    """z extent of ``atoms.centralize(mol)`` (the slab thickness the
+    reference sizes its slices from), without centring the other axes."""
+    z = np.asarray(mol.coordinates[:, 0], dtype=np.float32)
+    if z.size == 0:
+        return 0.0
+    lo, hi = z.min(), z.max()
+    zc = z + (-lo - (hi - lo) / np.float32(2.0))
+    return float(zc.max() - zc.min())
+
+
 class MixedBatch:
     """Molecules that need different wave sizes, batched by size.
 
@@ -99,20 +123,32 @@ class MixedBatch:
                                           box_size=(None, n, n))
         return max(int(ns), 1)
 
-    def run(self, items: Sequence[Tuple[atm.AtomList, int]], mode: int = 0) -> List[torch.Tensor]:
+    def run(self, items: Sequence[Tuple[atm.AtomList, int]], mode: int = 0,
+            device_prep: bool = True) -> List[torch.Tensor]:
         """items: (molecule, wave size); returns one (n, n) image (mode 0) or
-        complex64 exit wave (mode 1) per item, on the GPU, in input order."""
+        complex64 exit wave (mode 1) per item, on the GPU, in input order.
+
+        device_prep: molecules go to the GPU as they come and are centred and
+        grouped by element there (``BatchPipe.run_raw``); the host only finds
+        each molecule's thickness (its slice count) and element indices.
+        Otherwise they are centred and sorted on the host first
+        (``BatchPipe.prepare``).  Both give the same images."""
         out: List[torch.Tensor] = [None] * len(items)
         self.last_groups = []
-        # host preparation of every molecule on a thread pool
-        cen = pool_map(atm.centralize, [m for m, _ in items])
-        thick = pool_map(lambda m: float(m.space[0]), cen)
+        mols = [m for m, _ in items]
+        if device_prep:
+            thick = pool_map(centred_thickness, mols)
+        else:   # host preparation of every molecule on a thread pool
+            mols = pool_map(atm.centralize, mols)
+            thick = pool_map(lambda m: float(m.space[0]), mols)
         for n in sorted({int(sz) for _, sz in items}):
             idx = [i for i, (_, sz) in enumerate(items) if int(sz) == n]
             ns = self._slices_for(max(thick[i] for i in idx), n)
             p = self.pipe(n, ns)
-            batch = p.upload(p.prepare_many([cen[i] for i in idx]))
-            res = p._simulate(batch, mode)
+            if device_prep:
+                res = p._simulate_raw(p.upload_raw([mols[i] for i in idx]), mode)
+            else:
+                res = p._simulate(p.upload(p.prepare_many([mols[i] for i in idx])), mode)
             for j, i in enumerate(idx):
                 out[i] = res[j]
             self.last_groups.append((n, ns, idx))
@@ -189,6 +225,44 @@ class BatchPipe:
         counts = np.stack([m.counts for m in mols]).astype(np.int64)
         return PreparedBatch(as_device(coords, torch.float32, "coordinates"), mol_off,
                              np.ascontiguousarray(counts), [m.label for m in mols])
+
+    def element_indices(self, mol: atm.AtomList) -> np.ndarray:
+        """(n_atoms,) uint8 index of each atom's element in the pipe's list."""
+        z = np.asarray(mol.elements, dtype=np.int64)
+        lut = np.full(max(int(z.max(initial=0)), max(self.elements)) + 1, 255, dtype=np.uint8)
+        lut[self.elements] = np.arange(len(self.elements), dtype=np.uint8)
+        if z.size and (z.min() < 0 or (lut[z] == 255).any()):
+            unknown = sorted(set(np.unique(z).tolist()) - set(self.elements))
+            raise ValueError(f"molecule has elements {unknown} outside the pipe's element "
+                             f"list {self.elements}")
+        return lut[z]
+
+    def pack_raw(self, mols: Sequence[atm.AtomList]):
+        """Host side of ``upload_raw``: (coordinates (n, 3) float32 as given,
+        element indices (n,) uint8, molecule offsets, labels)."""
+        if len(mols) == 0:
+            raise ValueError("empty batch")
+        if len(self.elements) > 254:
+            raise ValueError("at most 254 elements per pipe")
+        sizes = np.array([len(m) for m in mols], dtype=np.int64)
+        mol_off = np.zeros(len(mols) + 1, dtype=np.int64)
+        np.cumsum(sizes, out=mol_off[1:])
+        elem = pool_map(self.element_indices, mols)
+        coords = np.concatenate([np.asarray(m.coordinates, dtype=np.float32).reshape(-1, 3)
+                                 for m in mols]) if mol_off[-1] else np.zeros((0, 3), np.float32)
+        elem = np.concatenate(elem) if mol_off[-1] else np.zeros(0, np.uint8)
+        return coords, np.ascontiguousarray(elem), mol_off, [getattr(m, "label", "") for m in mols]
+
+    def upload_packed(self, packed) -> RawBatch:
+        coords, elem, mol_off, labels = packed
+        return RawBatch(as_device(coords, torch.float32, "coordinates"),
+                        torch.from_numpy(elem).to(device()), mol_off, list(labels))
+
+    def upload_raw(self, mols: Sequence[atm.AtomList]) -> RawBatch:
+        """Molecules as they come, copied to the GPU once: coordinates as
+        given (not centred, atoms in any order) and per-atom element indices;
+        ``run_raw`` prepares them on the device."""
+        return self.upload_packed(self.pack_raw(mols))
 
     def upload_orientations(self, mol: atm.AtomList, quats: np.ndarray) -> PreparedBatch:
         """One molecule in len(quats) orientations, rotated and centred on the
@@ -271,6 +345,30 @@ class BatchPipe:
                      self.elements.index(8), float(voxel_water),
                      seeds.ctypes.data_as(POINTER(c_uint64)), *tail)
         return out
+
+    def _simulate_raw(self, raw: RawBatch, mode: int) -> torch.Tensor:
+        if self.add_water:
+            raise ValueError("run_raw: the water fill needs host-prepared molecules (run)")
+        B, n_atoms = len(raw), raw.coords.shape[0]
+        dtype = torch.float32 if mode == 0 else torch.complex64
+        out = torch.empty((B, self.n, self.n), dtype=dtype, device=device())
+        ws_bytes = hip_ext.raw("emsim_hip_simulate_raw_workspace_size")(
+            self.builder.backend.h, B, n_atoms, mode)
+        ws = self._buffer("simulate_ws", (max(ws_bytes, 1),), torch.uint8)
+        mic = self.microscope
+        psi0 = self.propagator.init_wave_value(mic.electron_dose)
+        hip_ext.call("emsim_hip_simulate_raw", self.builder.backend.h, self.propagator.backend.h,
+                     ptr(raw.coords), ptr(raw.elem), n_atoms,
+                     raw.mol_off.ctypes.data_as(POINTER(c_int64)), B, float(psi0),
+                     float(mic.cs_mm), float(mic.defocus), float(mic.aperture),
+                     1 if self.symmetric_bandlimit else 0, ptr(out), None, mode, ptr(ws),
+                     ws.numel(), stream_handle())
+        return out
+
+    def run_raw(self, raw: RawBatch) -> torch.Tensor:
+        """(B, n, n) float32 images of molecules uploaded as they come
+        (``upload_raw``): centred and grouped by element on the device."""
+        return self._simulate_raw(raw, 0)
 
     def run(self, batch: PreparedBatch, seeds=None) -> torch.Tensor:
         """(B, n, n) float32 images on the GPU (``seeds``: water seeds, used
@@ -435,9 +533,14 @@ class BatchSim:
     """Device-resident form of the reference's simulation loop
     (``emsim/simulator.py:21-91``).
 
-    A host thread prepares the molecules of ``mols`` (centre, sort by
-    element) and groups them into batches of ``batch``; the calling thread
-    uploads each batch once and runs ``BatchPipe.run`` on the current stream.
+    A host thread groups the molecules of ``mols`` into batches of ``batch``
+    and packs each batch's atoms with their element indices
+    (``BatchPipe.pack_raw``); the calling thread uploads each batch once and
+    runs ``BatchPipe.run_raw`` on the current stream, which centres the
+    molecules and groups their atoms by element on the device.  With
+    ``device_prep=False`` (and always for a pipe that adds water) the host
+    thread centres and sorts the molecules (``BatchPipe.prepare``) and the
+    batch runs through ``BatchPipe.run``; the images are the same.
     Each batch's images are copied to the host on a second stream (default
     priority) into a ring of ``depth`` pinned buffers, so the copy of batch
     k and the ``result_handler`` calls for the batch before it overlap the
@@ -447,10 +550,11 @@ class BatchSim:
     """
 
     def __init__(self, pipe: "BatchPipe", mols, result_handler, batch: int = 64,
-                 depth: int = 2):
+                 depth: int = 2, device_prep: bool = True):
         if depth < 1:
             raise ValueError("depth must be >= 1")
         self.pipe, self.mols, self.result_handler = pipe, mols, result_handler
+        self.device_prep = bool(device_prep) and not pipe.add_water
         self.batch, self.depth = int(batch), int(depth)
         self.delivered = 0
         self.failed = 0
@@ -461,6 +565,9 @@ class BatchSim:
         try:
             def attempt(mol):
                 try:
+                    if self.device_prep:
+                        self.pipe.element_indices(mol)   # refuses unknown elements
+                        return mol
                     return self.pipe.prepare(mol)
                 except Exception as e:   # reference: log the failed image, go on
                     return e
@@ -474,7 +581,7 @@ class BatchSim:
                     else:
                         prepared.append(res)
                 if prepared:
-                    q.put(prepared)
+                    q.put(self.pipe.pack_raw(prepared) if self.device_prep else prepared)
                 if stop.is_set():
                     break
         except Exception as e:   # an iterator that raises ends the stream
@@ -511,12 +618,15 @@ class BatchSim:
                 prepared = q.get()
                 if prepared is None:
                     break
-                labels = [m.label for m in prepared]
+                labels = prepared[3] if self.device_prep else [m.label for m in prepared]
                 try:
-                    images = self.pipe.run(self.pipe.upload(prepared))
+                    if self.device_prep:
+                        images = self.pipe.run_raw(self.pipe.upload_packed(prepared))
+                    else:
+                        images = self.pipe.run(self.pipe.upload(prepared))
                 except Exception as e:
-                    log.warning(f"batch of {len(prepared)} molecules failed: {e}")
-                    self.failed += len(prepared)
+                    log.warning(f"batch of {len(labels)} molecules failed: {e}")
+                    self.failed += len(labels)
                     continue
                 done = torch.cuda.Event()
                 done.record()
@@ -524,7 +634,7 @@ class BatchSim:
                     deliver()   # frees the ring slot about to be reused
                 copy_stream.wait_event(done)
                 with torch.cuda.stream(copy_stream):
-                    ring[slot][:len(prepared)].copy_(images, non_blocking=True)
+                    ring[slot][:len(labels)].copy_(images, non_blocking=True)
                     copied = torch.cuda.Event()
                     copied.record(copy_stream)
                 images.record_stream(copy_stream)

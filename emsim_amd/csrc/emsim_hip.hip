@@ -229,6 +229,7 @@ namespace {
 // They change how work is split over blocks, never the arithmetic.
 int g_row_items_per_block = 0;   // row groups per row-worker block
 int g_col_tiles_per_block = 0;   // column tiles per column-kernel block
+int g_col_shape = 0;             // column-tile shape (ColCfg SHAPE) of the propagator pass
 
 // Grid of the row worker for n_items rows (or row pairs) of length N.
 template <int N>
@@ -298,10 +299,10 @@ struct ColDead {
     bool skip_tiles = false;
 };
 
-template <int N, class Mul>
-int launch_col_mul_impl(float2* data, int ld, int ncols, int64_t img_stride, int n_img, Mul mul,
-                        const float2* tw, hipStream_t s, ColDead cd = ColDead{}) {
-    using Cfg = ColCfg<N>;
+template <int N, class Mul, int SHAPE = 0>
+int launch_col_mul_shape(float2* data, int ld, int ncols, int64_t img_stride, int n_img, Mul mul,
+                         const float2* tw, hipStream_t s, ColDead cd) {
+    using Cfg = ColCfg<N, SHAPE>;
     const int tiles_per_img = (int)ceil_div(ncols, Cfg::C);
     DeadCols dc;
     dc.lo = cd.lo;
@@ -322,15 +323,32 @@ int launch_col_mul_impl(float2* data, int ld, int ncols, int64_t img_stride, int
     const int64_t tpb = col_tiles_per_block(n_tiles);
     // blocks pair up on an XCD (see k_col_fft_mul_ifft): a multiple of 16
     const dim3 grid((unsigned)(ceil_div(ceil_div(n_tiles, tpb), 16) * 16));
-    EMSIM_LAUNCH(N, (k_col_fft_mul_ifft<N, Mul>), grid, dim3(Cfg::BLOCK), 0, s, data, ld, ncols,
-                 dc, img_stride, n_tiles, (int)tpb, mul, tw);
+    EMSIM_LAUNCH(N, (k_col_fft_mul_ifft<N, Mul, SHAPE>), grid, dim3(Cfg::BLOCK), 0, s, data, ld,
+                 ncols, dc, img_stride, n_tiles, (int)tpb, mul, tw);
     return EMSIM_HIP_OK;
+}
+
+// The propagator and last-slice passes of the fused path take the tuned tile
+// shape (emsim_hip_tuning_set("col_shape")) at the lengths that have them.
+template <int N, class Mul>
+int launch_col_mul_impl(float2* data, int ld, int ncols, int64_t img_stride, int n_img, Mul mul,
+                        const float2* tw, hipStream_t s, ColDead cd = ColDead{},
+                        bool tuned = false) {
+    if constexpr (col_shapes(N)) {
+        if (tuned && g_col_shape == 1)
+            return launch_col_mul_shape<N, Mul, 1>(data, ld, ncols, img_stride, n_img, mul, tw, s,
+                                                   cd);
+        if (tuned && g_col_shape == 2)
+            return launch_col_mul_shape<N, Mul, 2>(data, ld, ncols, img_stride, n_img, mul, tw, s,
+                                                   cd);
+    }
+    return launch_col_mul_shape<N, Mul, 0>(data, ld, ncols, img_stride, n_img, mul, tw, s, cd);
 }
 
 template <int N>
 int launch_col_prop(float2* data, int ld, int ncols, int64_t img_stride, int n_img,
                     PropagatorMul mul, const float2* tw, hipStream_t s, ColDead cd) {
-    return launch_col_mul_impl<N>(data, ld, ncols, img_stride, n_img, mul, tw, s, cd);
+    return launch_col_mul_impl<N>(data, ld, ncols, img_stride, n_img, mul, tw, s, cd, true);
 }
 
 template <int N>
@@ -1015,6 +1033,9 @@ extern "C" int emsim_hip_tuning_set(const char* key, int value) {
     } else if (std::strcmp(key, "col_tiles_per_block") == 0) {
         if (value < 0 || value > 32) return fail(EMSIM_HIP_EINVAL, "col_tiles_per_block in [0, 32]");
         g_col_tiles_per_block = value;
+    } else if (std::strcmp(key, "col_shape") == 0) {
+        if (value < 0 || value > 2) return fail(EMSIM_HIP_EINVAL, "col_shape in [0, 2]");
+        g_col_shape = value;
     } else {
         return fail(EMSIM_HIP_EINVAL, "tuning_set: unknown key '%s'", key);
     }
@@ -1024,6 +1045,7 @@ extern "C" int emsim_hip_tuning_set(const char* key, int value) {
 extern "C" int emsim_hip_tuning_get(const char* key) {
     if (key && std::strcmp(key, "row_items_per_block") == 0) return g_row_items_per_block;
     if (key && std::strcmp(key, "col_tiles_per_block") == 0) return g_col_tiles_per_block;
+    if (key && std::strcmp(key, "col_shape") == 0) return g_col_shape;
     return fail(EMSIM_HIP_EINVAL, "tuning_get: unknown key");
 }
 
@@ -1131,7 +1153,7 @@ int launch_slice_pass_a(bool first, const float2* Y, float2* wave, int64_t n_pai
 template <int N>
 int launch_col_lensprop(float2* data, int ld, int ncols, int64_t img_stride, int n_img,
                         LensPropMul mul, const float2* tw, hipStream_t s, ColDead cd) {
-    return launch_col_mul_impl<N>(data, ld, ncols, img_stride, n_img, mul, tw, s, cd);
+    return launch_col_mul_impl<N>(data, ld, ncols, img_stride, n_img, mul, tw, s, cd, true);
 }
 
 int spec_sparse(int n, const uint32_t* row_start, const uint16_t* bucket, const float* sf, float2* Y, int slice, int n_slices, int n_elems, int H, int ldh,
@@ -1168,6 +1190,11 @@ struct FusedWorkspace {
     float* hist;
     float2* X;
     uint64_t* seeds;
+    // molecules given as they come (emsim_hip_simulate_raw): per-atom element
+    // indices and bounding-box keys (batch, 6)
+    const uint8_t* atom_elem;
+    uint32_t* bbox;
+    int64_t max_mol_atoms;
     int64_t n_rows;
     size_t bytes;
 };
@@ -1180,7 +1207,7 @@ struct WaterFill {
 };
 
 FusedWorkspace fused_layout(const emsim_hip_slice_builder* b, int batch, int64_t n_atoms,
-                            bool own_wave, char* base, bool water = false) {
+                            bool own_wave, char* base, bool water = false, bool raw = false) {
     FusedWorkspace w;
     w.n_rows = (int64_t)batch * b->n_slices * b->n_elems * b->n1;
     const int64_t n_scan = w.n_rows + 1;
@@ -1213,6 +1240,13 @@ FusedWorkspace fused_layout(const emsim_hip_slice_builder* b, int batch, int64_t
         w.hist = (float*)(base + off);   off += align_up(sizeof(float) * planes * b->n2);
         w.X = (float2*)(base + off);     off += align_up(sizeof(float2) * planes * half_ld(b->H));
         w.seeds = (uint64_t*)(base + off); off += align_up(sizeof(uint64_t) * (size_t)batch);
+    }
+    w.atom_elem = nullptr;
+    w.bbox = nullptr;
+    w.max_mol_atoms = 0;
+    if (raw) {
+        w.bbox = (uint32_t*)(base + off);
+        off += align_up(sizeof(uint32_t) * 6 * (size_t)batch);
     }
     w.bytes = off;
     return w;
@@ -1377,6 +1411,44 @@ extern "C" int emsim_hip_simulate_water(const emsim_hip_slice_builder* b,
                          symmetric_bandlimit, out, slices_out, mode, s, wf);
 }
 
+extern "C" size_t emsim_hip_simulate_raw_workspace_size(const emsim_hip_slice_builder* b,
+                                                        int batch, int64_t n_atoms, int mode) {
+    if (!b || batch < 1 || n_atoms < 0) return 0;
+    return fused_layout(b, batch, n_atoms, mode == 0, nullptr, false, true).bytes;
+}
+
+extern "C" int emsim_hip_simulate_raw(const emsim_hip_slice_builder* b,
+                                      const emsim_hip_wave_propagator* w, const float* coords,
+                                      const uint8_t* elem, int64_t n_atoms,
+                                      const int64_t* mol_off_host, int batch, float psi0,
+                                      float cs_mm, float defocus, float aperture,
+                                      int symmetric_bandlimit, float* out, float* slices_out,
+                                      int mode, void* workspace, size_t workspace_bytes,
+                                      emsim_stream_t stream_) {
+    hipStream_t s = (hipStream_t)stream_;
+    if (!mol_off_host || (n_atoms > 0 && !elem))
+        return fail(EMSIM_HIP_EINVAL, "simulate_raw: null offsets or element indices");
+    int rc = check_simulate(b, w, coords, n_atoms, batch, symmetric_bandlimit, out, mode);
+    if (rc) return rc;
+    if (mol_off_host[0] != 0 || mol_off_host[batch] != n_atoms)
+        return fail(EMSIM_HIP_EINVAL, "simulate_raw: molecule offsets must run from 0 to n_atoms");
+    int64_t max_atoms = 0;
+    for (int m = 0; m < batch; ++m) {
+        const int64_t len = mol_off_host[m + 1] - mol_off_host[m];
+        if (len < 0) return fail(EMSIM_HIP_EINVAL, "simulate_raw: offsets must not decrease");
+        if (len > max_atoms) max_atoms = len;
+    }
+    FusedWorkspace ws = fused_layout(b, batch, n_atoms, mode == 0, (char*)workspace, false, true);
+    if (!workspace || workspace_bytes < ws.bytes)
+        return fail(EMSIM_HIP_EWORKSPACE, "simulate_raw: workspace needs %zu bytes", ws.bytes);
+    ws.atom_elem = elem;
+    ws.max_mol_atoms = max_atoms;
+    EMSIM_HIP_CHECK(hipMemcpyAsync(ws.mol_off, mol_off_host, sizeof(int64_t) * (batch + 1),
+                                   hipMemcpyHostToDevice, s));
+    return simulate_core(b, w, coords, n_atoms, ws, batch, psi0, cs_mm, defocus, aperture,
+                         symmetric_bandlimit, out, slices_out, mode, s);
+}
+
 namespace {
 
 int simulate_core(const emsim_hip_slice_builder* b, const emsim_hip_wave_propagator* w,
@@ -1408,9 +1480,19 @@ int simulate_core(const emsim_hip_slice_builder* b, const emsim_hip_wave_propaga
     const int64_t n_blocks = ceil_div(n_scan, kScanTile);
     EMSIM_LAUNCH(0, k_fill_u32, dim3(grid_for(n_scan, 256)), dim3(256), 0, s, ws.row_count,
                  n_scan, 0u);
+    if (ws.atom_elem && n_atoms > 0) {
+        // molecules as they come: their bounding boxes, for centring on the fly
+        EMSIM_LAUNCH(0, k_bbox_init, dim3((unsigned)ceil_div(6 * (int64_t)batch, 256)), dim3(256),
+                     0, s, ws.bbox, batch);
+        const int64_t chunks = ceil_div(ws.max_mol_atoms > 0 ? ws.max_mol_atoms : 1,
+                                        256 * kBoxPerThread);
+        EMSIM_LAUNCH(0, k_mol_bbox, dim3((unsigned)chunks, (unsigned)batch), dim3(256), 0, s,
+                     coords, (const int64_t*)ws.mol_off, ws.atom_elem, b->n_elems, ws.bbox);
+    }
     if (n_atoms > 0)
         EMSIM_LAUNCH(0, k_csr_classify, dim3(grid_for(n_atoms, 256)), dim3(256), 0, s, coords,
-                     n_atoms, ws.mol_off, ws.elem_off, bg, ws.atom_row, ws.atom_y, ws.row_count);
+                     n_atoms, ws.mol_off, ws.elem_off, bg, ws.atom_row, ws.atom_y, ws.row_count,
+                     ws.atom_elem, (const uint32_t*)ws.bbox);
     EMSIM_LAUNCH(0, k_scan_tiles, dim3((unsigned)n_blocks), dim3(kScanBlock), 0, s, ws.row_count,
                  ws.row_start, n_scan, ws.block_sums);
     EMSIM_LAUNCH(0, k_scan_sums, dim3(1), dim3(kScanBlock), 0, s, ws.block_sums, n_blocks);

@@ -26,6 +26,7 @@
 #include <stdint.h>
 
 #include "fft_lds.hpp"
+#include "kernels_prep.hpp"
 #include "kernels_slice.hpp"
 #include "kernels_wave.hpp"
 
@@ -39,12 +40,18 @@ struct BatchGeom {
 
 // Pass 1: per atom its row and y bin; count atoms per row.  mol_off (B+1)
 // and elem_off (B (nE+1)) are absolute atom offsets of molecules and of each
-// molecule's element groups.
+// molecule's element groups.  Molecules given as they come instead
+// (atom_elem non-null): atom a's element index is atom_elem[a] (n_elems or
+// more: not binned), elem_off is unused, and coordinates are centred on the
+// fly with the molecule's bounding-box keys bbox (B, 6) -- the float32
+// shift atoms.centralize applies, so the bins equal the centred host copy's.
 __global__ void k_csr_classify(const float* __restrict__ coords, int64_t n_atoms,
                                const int64_t* __restrict__ mol_off,
                                const int64_t* __restrict__ elem_off, BatchGeom bg,
                                int32_t* __restrict__ atom_row, uint16_t* __restrict__ atom_y,
-                               uint32_t* __restrict__ row_count) {
+                               uint32_t* __restrict__ row_count,
+                               const uint8_t* __restrict__ atom_elem = nullptr,
+                               const uint32_t* __restrict__ bbox = nullptr) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const BinGeom& g = bg.g;
     for (int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; a < n_atoms; a += stride) {
@@ -55,13 +62,27 @@ __global__ void k_csr_classify(const float* __restrict__ coords, int64_t n_atoms
             if (mol_off[mid] <= a) lo = mid; else hi = mid;
         }
         const int b = lo;
-        const int64_t* eo = elem_off + (int64_t)b * (bg.n_elems + 1);
         int32_t row = -1;
         uint16_t yb = 0;
-        if (a >= eo[0] && a < eo[bg.n_elems]) {
-            int e = 0;
-            while (e + 1 < bg.n_elems && a >= eo[e + 1]) ++e;
-            const float z = coords[3 * a + 0], x = coords[3 * a + 1], y = coords[3 * a + 2];
+        int e = -1;
+        if (atom_elem) {
+            const int ea = atom_elem[a];
+            if (ea < bg.n_elems) e = ea;
+        } else {
+            const int64_t* eo = elem_off + (int64_t)b * (bg.n_elems + 1);
+            if (a >= eo[0] && a < eo[bg.n_elems]) {
+                e = 0;
+                while (e + 1 < bg.n_elems && a >= eo[e + 1]) ++e;
+            }
+        }
+        if (e >= 0) {
+            float z = coords[3 * a + 0], x = coords[3 * a + 1], y = coords[3 * a + 2];
+            if (atom_elem) {
+                const uint32_t* bb = bbox + 6 * (int64_t)b;
+                z = __fadd_rn(z, centring_shift(bb, 0));
+                x = __fadd_rn(x, centring_shift(bb, 1));
+                y = __fadd_rn(y, centring_shift(bb, 2));
+            }
             // a one-slice builder bins in 2-D (x, y) and ignores z
             const int s = g.one_slice ? 0 : bin_of(z, g.e0, g.n0, g.last0, g.inv0);
             const int i = bin_of(x, g.e1, g.n1, g.last1, g.inv1);

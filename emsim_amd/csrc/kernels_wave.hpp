@@ -125,20 +125,29 @@ struct RowCfg {
 #ifndef EMSIM_COL_MIN_WAVES
 #define EMSIM_COL_MIN_WAVES 1
 #endif
-template <int N>
+// SHAPE selects the tile shape (0: the built-in rule; the others are the
+// alternatives the launch-shape autotuner may pick, see kColShapes):
+//   1: EPT 8, half the columns per block (smaller blocks, more of them per CU)
+//   2: EPT 16 (two radix-8 butterflies per thread and pass, half the threads
+//      per column, the same columns per block)
+template <int N, int SHAPE = 0>
 struct ColCfg {
-    static constexpr int EPT = (N / EMSIM_COL_EPT) > 256 ? N / 256
-                               : (EMSIM_COL_EPT < N ? EMSIM_COL_EPT : 8);
+    static constexpr int EPT0 = (N / EMSIM_COL_EPT) > 256 ? N / 256
+                                : (EMSIM_COL_EPT < N ? EMSIM_COL_EPT : 8);
+    static constexpr int EPT = SHAPE == 2 ? 2 * EPT0 : EPT0;
     static constexpr int T = N / EPT;                        // threads per column
     static constexpr int STRIDE = LdsLayout<N>::kStride;     // odd: columns on distinct banks
     // threads per block, and 96 KiB of exchange planes (the rest of the
     // 160 KiB LDS holds the propagator's row table)
-    static constexpr int CMAX_T = EMSIM_COL_MAX_BLOCK / T;
+    static constexpr int CMAX_T = EMSIM_COL_MAX_BLOCK / (N / EPT0);
     static constexpr int CMAX_L = (96 * 1024) / (8 * STRIDE);
     static constexpr int CMAX = CMAX_T < CMAX_L ? CMAX_T : CMAX_L;
-    static constexpr int C = CMAX < 1 ? 1 : (CMAX < 16 ? CMAX : 16);
+    static constexpr int C0 = CMAX < 1 ? 1 : (CMAX < 16 ? CMAX : 16);
+    static constexpr int C = SHAPE == 1 ? (C0 > 1 ? C0 / 2 : 1) : C0;
     static constexpr int BLOCK = T * C;
 };
+// lengths with alternative column-tile shapes (the others use shape 0)
+constexpr bool col_shapes(int n) { return n == 256 || n == 512 || n == 1024; }
 
 // Index of this thread's transform within the block; wave-uniform when a
 // transform spans whole waves, so it lives in a scalar register.
@@ -348,11 +357,18 @@ struct DeadCols {
 template <int N>
 constexpr bool col_db() { return kColDoubleBuffer && fft_exchanges(N) % 2 == 0 && N >= 64; }
 
-template <int N, class Mul>
-__global__ __launch_bounds__(ColCfg<N>::BLOCK, col_db<N>() ? 4 : EMSIM_COL_MIN_WAVES) void k_col_fft_mul_ifft(
+// waves per SIMD a column kernel leaves room for: 128 VGPRs with alternating
+// plane sets at 8 values per thread; 16 values per thread need more
+template <int N, int SHAPE>
+constexpr int col_min_waves() {
+    return SHAPE == 2 ? 2 : (col_db<N>() ? 4 : EMSIM_COL_MIN_WAVES);
+}
+
+template <int N, class Mul, int SHAPE = 0>
+__global__ __launch_bounds__((ColCfg<N, SHAPE>::BLOCK), (col_min_waves<N, SHAPE>())) void k_col_fft_mul_ifft(
     float2* data, int ld, int ncols, DeadCols dc, int64_t img_stride, int64_t n_tiles,
     int tpb, Mul mul, const float2* __restrict__ tw) {
-    using Cfg = ColCfg<N>;
+    using Cfg = ColCfg<N, SHAPE>;
     constexpr int EPT = Cfg::EPT;
     // a second plane set where the exchange count allows alternating sets
     // (one barrier per exchange instead of two)

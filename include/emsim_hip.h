@@ -210,6 +210,26 @@ int emsim_hip_simulate_water(const emsim_hip_slice_builder* b, const emsim_hip_w
                              float* out_dev, float* slices_out_dev, int mode, void* workspace_dev,
                              size_t workspace_bytes, emsim_stream_t stream);
 
+/* emsim_hip_simulate for molecules as they come, with the host preparation
+ * (emsim/pipe.py:47 atoms.centralize, emsim/pot.py:40-41
+ * atoms.sort_elements_and_count, atoms.py:53-86,123-159) done on the device:
+ * coords_dev holds each molecule's atoms uncentred and in any order,
+ * elem_dev (n_atoms) the index of each atom's element in the builder's
+ * element list (n_elems or more: the atom is not simulated).  Each
+ * molecule's bounding box is reduced on the device and its atoms are
+ * centred on the fly with the float32 shift atoms.centralize applies, so
+ * the images equal emsim_hip_simulate's on the host-prepared copy.  No
+ * per-element offsets are needed: atoms are grouped by element as they are
+ * bucketed. */
+size_t emsim_hip_simulate_raw_workspace_size(const emsim_hip_slice_builder* b, int batch,
+                                             int64_t n_atoms, int mode);
+int emsim_hip_simulate_raw(const emsim_hip_slice_builder* b, const emsim_hip_wave_propagator* w,
+                           const float* coords_dev, const uint8_t* elem_dev, int64_t n_atoms,
+                           const int64_t* mol_off_host, int batch, float psi0, float cs_mm,
+                           float defocus, float aperture, int symmetric_bandlimit, float* out_dev,
+                           float* slices_out_dev, int mode, void* workspace_dev,
+                           size_t workspace_bytes, emsim_stream_t stream);
+
 /* ---- learned coarse-slice correction ------------------------------------
  * No counterpart in the reference (emsim has no learned component).  A
  * Fourier neural operator (lifting, `layers` spectral layers of `channels`

@@ -17,11 +17,12 @@ def _fake(costs):
 
 
 def test_coordinate_descent_finds_separable_optimum():
-    # cost minimised at row=4, col=2 (separable)
-    costs = lambda s: (s["row_items_per_block"] - 4) ** 2 + (s["col_tiles_per_block"] - 2) ** 2 + 1
+    # cost minimised at row=4, col=2, shape=2 (separable)
+    costs = lambda s: ((s["row_items_per_block"] - 4) ** 2 + (s["col_tiles_per_block"] - 2) ** 2
+                       + (s["col_shape"] - 2) ** 2 + 1)
     set_knobs, timer, state = _fake(costs)
     best, table = autotune.tune(lambda: None, sync=lambda: None, set_knobs=set_knobs, timer=timer)
-    assert best == {"row_items_per_block": 4, "col_tiles_per_block": 2}
+    assert best == {"row_items_per_block": 4, "col_tiles_per_block": 2, "col_shape": 2}
     assert state == best                       # the winner is left applied
     assert len(table) == sum(len(v) for v in autotune.KNOBS.values())
 
@@ -29,7 +30,7 @@ def test_coordinate_descent_finds_separable_optimum():
 def test_default_rule_kept_when_nothing_is_faster():
     set_knobs, timer, _ = _fake(lambda s: 1.0 + sum(v > 0 for v in s.values()))
     best, _ = autotune.tune(lambda: None, sync=lambda: None, set_knobs=set_knobs, timer=timer)
-    assert best == {"row_items_per_block": 0, "col_tiles_per_block": 0}
+    assert best == {"row_items_per_block": 0, "col_tiles_per_block": 0, "col_shape": 0}
 
 
 def test_cache_roundtrip(tmp_path):

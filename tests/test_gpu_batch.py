@@ -287,18 +287,21 @@ def test_graph_replay_equals_eager_and_follows_new_coords():
     assert not torch.equal(out2, eager)
 
 
-def test_launch_shapes_do_not_change_results():
+@pytest.mark.parametrize("n", [128, 256])
+def test_launch_shapes_do_not_change_results(n):
+    """Every launch-shape setting, column-tile shapes included (256 has the
+    alternative shapes), gives bit-identical images."""
     from emsim_amd import autotune
     tem = _tem()
     mols = _mols([3000, 5000], 81)
-    pipe = BatchPipe(tem, 1.0, 2.0, 128, 8, ELEMS)
+    pipe = BatchPipe(tem, 1.0, 2.0, n, 8, ELEMS)
     batch = pipe.upload([pipe.prepare(m) for m in mols])
     autotune.apply({})
     ref = pipe.run(batch).clone()
     try:
-        for cfg in ({"row_items_per_block": 1, "col_tiles_per_block": 1},
-                    {"row_items_per_block": 7, "col_tiles_per_block": 5},
-                    {"row_items_per_block": 16, "col_tiles_per_block": 8}):
+        for cfg in ({"row_items_per_block": 1, "col_tiles_per_block": 1, "col_shape": 1},
+                    {"row_items_per_block": 7, "col_tiles_per_block": 5, "col_shape": 2},
+                    {"row_items_per_block": 16, "col_tiles_per_block": 8, "col_shape": 0}):
             autotune.apply(cfg)
             assert autotune.current() == cfg
             assert torch.equal(pipe.run(batch), ref)

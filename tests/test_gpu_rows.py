@@ -249,3 +249,31 @@ def test_batch_pipe_water_matches_pipe_and_oracle():
     other = _host(bp.run(batch, seeds=[seeds[0] + 1, seeds[1]]))
     assert orc.rel_l2(other[0], images[0]) > 1e-3
     np.testing.assert_array_equal(other[1], images[1])
+
+
+def test_device_preparation_equals_host_preparation():
+    """Molecules uploaded as they come (not centred, atoms in random order,
+    run_raw) give the images of the host-prepared copies (centralize + sort
+    by element, run) bit for bit, alone and through MixedBatch; an atom of an
+    element outside the pipe's list is refused on the host."""
+    tem = _tem()
+    quats = random_uniform_quaternions(4, rng=np.random.default_rng(331))
+    rng = np.random.default_rng(332)
+    mols = []
+    for i, n in enumerate((9000, 3000, 15000, 6000)):
+        m = synth.rotated(synth.protein_cloud(n, 330 + i), quats[i])
+        perm = rng.permutation(len(m))                       # any atom order
+        shift = rng.uniform(-40, 40, 3).astype(np.float32)   # not centred
+        mols.append(atm.AtomList(m.elements[perm], m.coordinates[perm] + shift))
+    bp = BatchPipe(tem, 1.0, 2.0, 128, 20, ELEMS)
+    host = bp.run(bp.upload([bp.prepare(m) for m in mols]))
+    dev = bp.run_raw(bp.upload_raw(mols))
+    assert torch.equal(host, dev)
+    items = [(m, n) for m, n in zip(mols, (128, 256, 128, 256))]
+    mb = MixedBatch(tem, 1.0, 2.0, ELEMS)
+    a = mb.run(items, device_prep=True)
+    b = mb.run(items, device_prep=False)
+    assert all(torch.equal(x, y) for x, y in zip(a, b))
+    bad = atm.AtomList(np.array([6, 79]), np.zeros((2, 3), np.float32))
+    with pytest.raises(ValueError, match="outside"):
+        bp.upload_raw([mols[0], bad])
