@@ -5,6 +5,7 @@
 // kernels in kernels_wave.hpp / kernels_slice.hpp on the caller's stream.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -21,6 +22,9 @@
 #include "kernels_wave.hpp"
 #include "kernels_fused.hpp"
 #include "kernels_prep.hpp"
+#include "kernels_dft.hpp"
+
+#include <complex>
 
 using namespace emsim;
 
@@ -97,6 +101,99 @@ namespace {
 constexpr float kPi32 = 3.14159265358979324f;
 
 bool pow2_supported(int n) { return n >= 8 && n <= kTwN && (n & (n - 1)) == 0; }
+// Lengths the chirp-z path takes: any n with a power-of-two convolution
+// length 2n - 1 <= M <= kTwN.
+constexpr int kChirpMaxN = kTwN / 2;
+bool chirp_supported(int n) { return n >= 8 && n <= kChirpMaxN; }
+bool size_supported(int n) { return pow2_supported(n) || chirp_supported(n); }
+
+// --------------------------------------------------------------------------
+// chirp-z plans (kernels_dft.hpp): c_m = exp(-i pi m^2 / n) and the transform
+// of the circular chirp b, both computed in float64 on the host
+// --------------------------------------------------------------------------
+struct ChirpPlan {
+    int n = 0, M = 0;
+    float2* c_fwd = nullptr;   // (n) exp(-i pi m^2 / n)
+    float2* c_inv = nullptr;   // (n) conj
+    float2* b_fwd = nullptr;   // (M) FFT_M of conj(c_fwd) placed at m and M - m
+    float2* b_inv = nullptr;   // (M) the same for the inverse transform
+};
+
+// In-place iterative radix-2 FFT in float64, forward sign exp(-2 pi i j k / M).
+void host_fft(std::vector<std::complex<double>>& a) {
+    const size_t m = a.size();
+    for (size_t i = 1, j = 0; i < m; ++i) {
+        size_t bit = m >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) std::swap(a[i], a[j]);
+    }
+    const double two_pi = 6.283185307179586476925286766559;
+    for (size_t len = 2; len <= m; len <<= 1) {
+        for (size_t i = 0; i < m; i += len) {
+            for (size_t k = 0; k < len / 2; ++k) {
+                const double ang = -two_pi * (double)k / (double)len;
+                const std::complex<double> w(std::cos(ang), std::sin(ang));
+                const std::complex<double> u = a[i + k], v = a[i + k + len / 2] * w;
+                a[i + k] = u + v;
+                a[i + k + len / 2] = u - v;
+            }
+        }
+    }
+}
+
+void chirp_plan_free(ChirpPlan& p) {
+    for (float2** q : {&p.c_fwd, &p.c_inv, &p.b_fwd, &p.b_inv}) {
+        if (*q) (void)hipFree(*q);
+        *q = nullptr;
+    }
+}
+
+int chirp_plan_create(int n, ChirpPlan& p) {
+    int M = 1;
+    while (M < 2 * n - 1) M <<= 1;
+    if (M < 8) M = 8;
+    if (M > kTwN) return fail(EMSIM_HIP_EUNSUPPORTED, "chirp-z length %d needs M = %d > %d", n, M, kTwN);
+    p.n = n;
+    p.M = M;
+    const double pi = 3.14159265358979323846264338327950288;
+    std::vector<std::complex<double>> c(n);
+    for (int m = 0; m < n; ++m) {
+        const long long q = ((long long)m * m) % (2LL * n);   // m^2 mod 2n: exact angle
+        c[m] = std::polar(1.0, -pi * (double)q / (double)n);
+    }
+    std::vector<float2> hc(n), hci(n), hb(M), hbi(M);
+    for (int m = 0; m < n; ++m) {
+        hc[m] = make_float2((float)c[m].real(), (float)c[m].imag());
+        hci[m] = make_float2((float)c[m].real(), (float)-c[m].imag());
+    }
+    for (int dir = 0; dir < 2; ++dir) {
+        // the convolution kernel: conj of the transform's chirp, circular
+        std::vector<std::complex<double>> b(M, 0.0);
+        for (int m = 0; m < n; ++m) {
+            const std::complex<double> v = dir == 0 ? std::conj(c[m]) : c[m];
+            b[m] = v;
+            if (m > 0) b[M - m] = v;
+        }
+        host_fft(b);
+        std::vector<float2>& dst = dir == 0 ? hb : hbi;
+        for (int k = 0; k < M; ++k) dst[k] = make_float2((float)b[k].real(), (float)b[k].imag());
+    }
+    hipError_t err = hipSuccess;
+    const std::vector<float2>* src[4] = {&hc, &hci, &hb, &hbi};
+    float2** dst[4] = {&p.c_fwd, &p.c_inv, &p.b_fwd, &p.b_inv};
+    for (int i = 0; i < 4 && err == hipSuccess; ++i) {
+        err = hipMalloc(dst[i], src[i]->size() * sizeof(float2));
+        if (err == hipSuccess)
+            err = hipMemcpy(*dst[i], src[i]->data(), src[i]->size() * sizeof(float2),
+                            hipMemcpyHostToDevice);
+    }
+    if (err != hipSuccess) {
+        chirp_plan_free(p);
+        return fail(EMSIM_HIP_EDEVICE, "chirp plan: %s", hipGetErrorString(err));
+    }
+    return EMSIM_HIP_OK;
+}
 
 // --------------------------------------------------------------------------
 // per-device twiddle table w[m] = exp(-2 pi i m / 4096), float64 -> float32
@@ -190,6 +287,9 @@ struct emsim_hip_slice_builder {
     float inv_d[3] = {0, 0, 0};
     int tile_rows = 1, tiles_per_plane = 1;
     const float2* tw = nullptr;
+    // a lateral size that is not a power of two: chirp-z plans of both axes
+    bool chirp = false;
+    ChirpPlan p1, p2;
 };
 
 namespace {
@@ -203,6 +303,11 @@ struct emsim_hip_wave_propagator {
     float d1 = 0, d2 = 0, lam = 0, gamma = 0;
     float dfx = 0, dfy = 0, filter2 = 0, factor = 0;
     const float2* tw = nullptr;
+    // a wave shape that is not a power of two: chirp-z plans of both axes and
+    // the line buffer of one image (max(n1 M2, n2 M1) complex64)
+    bool chirp = false;
+    ChirpPlan p1, p2;
+    float2* work = nullptr;
 };
 
 // ==========================================================================
@@ -498,6 +603,76 @@ LensMul make_lens(const emsim_hip_wave_propagator* w, float cs_mm, float defocus
     return l;
 }
 
+// --------------------------------------------------------------------------
+// chirp-z transforms of lines and images (sizes that are not powers of two)
+// --------------------------------------------------------------------------
+constexpr int64_t kOneImage = (int64_t)1 << 62;   // LineMap::per_img of a single image
+
+// `lines` lines of p.n values (read from in through im in MODE, written to
+// out through om): their unnormalised DFT (inv: exp(+2 pi i j k / n)),
+// values k < n_out stored; real_out: real parts clipped at clip_lo.
+// work: lines x p.M complex64.
+int line_dft(const ChirpPlan& p, bool inv, int mode, const void* in, LineMap im, int64_t lines,
+             void* out, LineMap om, int n_out, bool real_out, float clip_lo, float2* work,
+             const float2* tw, hipStream_t s) {
+    if (lines <= 0) return EMSIM_HIP_OK;
+    const float2* c = inv ? p.c_inv : p.c_fwd;
+    const float2* B = inv ? p.b_inv : p.b_fwd;
+    const int64_t total = lines * (int64_t)p.M;
+    const dim3 gi(grid_for(total, 256)), blk(256);
+    if (mode == kReal)
+        EMSIM_LAUNCH(p.n, k_chirp_in<kReal>, gi, blk, 0, s, in, im, lines, p.n, p.M, c, work);
+    else if (mode == kHalf)
+        EMSIM_LAUNCH(p.n, k_chirp_in<kHalf>, gi, blk, 0, s, in, im, lines, p.n, p.M, c, work);
+    else
+        EMSIM_LAUNCH(p.n, k_chirp_in<kComplex>, gi, blk, 0, s, in, im, lines, p.n, p.M, c, work);
+    int rc = row_fft(p.M, false, work, work, lines, tw, s);
+    if (rc) return rc;
+    EMSIM_LAUNCH(p.n, k_line_mul, gi, blk, 0, s, work, lines, p.M, B);
+    rc = row_fft(p.M, true, work, work, lines, tw, s);
+    if (rc) return rc;
+    const float scale = 1.0f / (float)p.M;   // exact: M is a power of two
+    const dim3 go(grid_for(lines * (int64_t)n_out, 256));
+    if (real_out)
+        EMSIM_LAUNCH(p.n, k_chirp_out<true>, go, blk, 0, s, (const float2*)work, lines, n_out,
+                     p.M, c, scale, clip_lo, out, om);
+    else
+        EMSIM_LAUNCH(p.n, k_chirp_out<false>, go, blk, 0, s, (const float2*)work, lines, n_out,
+                     p.M, c, scale, clip_lo, out, om);
+    return EMSIM_HIP_OK;
+}
+
+// Line maps of one n1 x n2 image with rows ld apart: its rows, its columns.
+LineMap rows_of(int ld) { return LineMap{kOneImage, 0, ld, 1}; }
+LineMap cols_of(int ld) { return LineMap{kOneImage, 0, 1, ld}; }
+
+// Unnormalised 2-D DFT of one n1 x n2 complex image (out may be in).
+int chirp_dft2(const emsim_hip_wave_propagator* w, bool inv, const float2* in, float2* out,
+               hipStream_t s) {
+    int rc = line_dft(w->p2, inv, kComplex, in, rows_of(w->n2), w->n1, out, rows_of(w->n2),
+                      w->n2, false, 0.f, w->work, w->tw, s);
+    if (rc) return rc;
+    return line_dft(w->p1, inv, kComplex, out, cols_of(w->n2), w->n2, out, cols_of(w->n2),
+                    w->n1, false, 0.f, w->work, w->tw, s);
+}
+
+// FFT -> Fourier-space multiplier -> IFFT of every image of a batch.
+template <class Mul>
+int chirp_filter(const emsim_hip_wave_propagator* w, const float2* in, float2* out, int batch,
+                 const Mul& mul, hipStream_t s) {
+    const int64_t npix = (int64_t)w->n1 * w->n2;
+    for (int b = 0; b < batch; ++b) {
+        float2* o = out + b * npix;
+        int rc = chirp_dft2(w, false, in + b * npix, o, s);
+        if (rc) return rc;
+        EMSIM_LAUNCH(0, k_spectrum_mul<Mul>, dim3(grid_for(npix, 256)), dim3(256), 0, s, o, npix,
+                     w->n1, w->n2, mul);
+        rc = chirp_dft2(w, true, o, o, s);
+        if (rc) return rc;
+    }
+    return EMSIM_HIP_OK;
+}
+
 }  // namespace
 
 // ==========================================================================
@@ -507,7 +682,9 @@ extern "C" int emsim_hip_abi_version(void) { return EMSIM_HIP_ABI_VERSION; }
 
 extern "C" const char* emsim_hip_last_error(void) { return g_last_error.c_str(); }
 
-extern "C" int emsim_hip_fft_size_supported(int n) { return pow2_supported(n) ? 1 : 0; }
+extern "C" int emsim_hip_fft_size_supported(int n) {
+    return pow2_supported(n) ? 1 : (chirp_supported(n) ? 2 : 0);
+}
 
 extern "C" int emsim_hip_bin_edges(int n, double d, double* edges64_host, float* edges32_host,
                                    int cap, int* len_out) {
@@ -536,10 +713,10 @@ extern "C" int emsim_hip_slice_builder_create(emsim_hip_slice_builder** out,
                                       "and a scattering-factor table");
     if (n_elems > kMaxElems)
         return fail(EMSIM_HIP_EUNSUPPORTED, "slice_builder_create: at most %d elements", kMaxElems);
-    if (!pow2_supported(n1) || !pow2_supported(n2))
+    if (!size_supported(n1) || !size_supported(n2))
         return fail(EMSIM_HIP_EUNSUPPORTED,
                     "slice_builder_create: lateral size %dx%d; the hip backend implements "
-                    "powers of two from 8 to %d", n1, n2, kTwN);
+                    "sizes from 8 to %d (powers of two up to %d)", n1, n2, kChirpMaxN, kTwN);
     if (one_slice && n_slices != 1)
         return fail(EMSIM_HIP_EINVAL, "slice_builder_create: one-slice builder needs n_slices == 1");
     if (!(d1 > 0) || !(d2 > 0) || (!one_slice && !(dz > 0)))
@@ -594,8 +771,13 @@ extern "C" int emsim_hip_slice_builder_create(emsim_hip_slice_builder** out,
     b->tiles_per_plane = (int)ceil_div(n1, rows);
 
     int rc = get_twiddles(&b->tw);
+    if (!rc && !(pow2_supported(n1) && pow2_supported(n2))) {
+        b->chirp = true;
+        rc = chirp_plan_create(n1, b->p1);
+        if (!rc) rc = chirp_plan_create(n2, b->p2);
+    }
     if (rc) {
-        delete b;
+        emsim_hip_slice_builder_destroy(b);
         return rc;
     }
     const size_t sf_bytes = (size_t)n_elems * n1 * b->H * sizeof(float);
@@ -615,6 +797,8 @@ extern "C" int emsim_hip_slice_builder_create(emsim_hip_slice_builder** out,
 
 extern "C" int emsim_hip_slice_builder_destroy(emsim_hip_slice_builder* b) {
     if (!b) return EMSIM_HIP_OK;
+    chirp_plan_free(b->p1);
+    chirp_plan_free(b->p2);
     if (b->sf) (void)hipFree(b->sf);
     if (b->edges) (void)hipFree(b->edges);
     delete b;
@@ -737,11 +921,23 @@ extern "C" int emsim_hip_add_water(const emsim_hip_slice_builder* b, float* hist
     return EMSIM_HIP_OK;
 }
 
+namespace {
+// Workspace of make_slices per slice of a chunk: the half spectra of every
+// element, plus (chirp-z sizes) the line buffer of one element's transforms.
+size_t slices_bytes_per_slice(const emsim_hip_slice_builder* b) {
+    size_t per = (size_t)b->n_elems * b->n1 * half_ld(b->H) * sizeof(float2);
+    if (b->chirp)
+        per += sizeof(float2) * (size_t)std::max((int64_t)b->n1 * b->p2.M,
+                                                 (int64_t)b->H * b->p1.M);
+    return per;
+}
+}  // namespace
+
 extern "C" size_t emsim_hip_slices_workspace_size(const emsim_hip_slice_builder* b,
                                                   int max_chunk) {
     if (!b) return 0;
     int chunk = (max_chunk <= 0 || max_chunk > b->n_slices) ? b->n_slices : max_chunk;
-    return (size_t)b->n_elems * chunk * b->n1 * half_ld(b->H) * sizeof(float2);
+    return slices_bytes_per_slice(b) * (size_t)chunk;
 }
 
 extern "C" int emsim_hip_make_slices(const emsim_hip_slice_builder* b, const float* hist,
@@ -754,7 +950,7 @@ extern "C" int emsim_hip_make_slices(const emsim_hip_slice_builder* b, const flo
                     "make_slices: the symmetric band-limit needs a square grid (the reference's "
                     "mask does not broadcast for %dx%d)", b->n1, b->n2);
     const int ldh = half_ld(b->H);
-    const size_t per_slice = (size_t)b->n_elems * b->n1 * ldh * sizeof(float2);
+    const size_t per_slice = slices_bytes_per_slice(b);
     if (!workspace || workspace_bytes < per_slice)
         return fail(EMSIM_HIP_EWORKSPACE, "make_slices: workspace needs at least %zu bytes",
                     per_slice);
@@ -764,6 +960,40 @@ extern "C" int emsim_hip_make_slices(const emsim_hip_slice_builder* b, const flo
     float2* X = (float2*)workspace;
     const int r = (b->n1 < b->n2 ? b->n1 : b->n2) / 2;
     const float inv_npix = 1.0f / (float)((int64_t)b->n1 * b->n2);
+    if (b->chirp) {
+        // the same passes through chirp-z transforms: R2C rows, column DFTs,
+        // spectrum sum, inverse column DFTs, C2R rows with the clip
+        float2* work = X + (size_t)b->n_elems * chunk * b->n1 * ldh;
+        const int n1 = b->n1, n2 = b->n2, H = b->H, nE = b->n_elems;
+        for (int s0 = 0; s0 < b->n_slices; s0 += chunk) {
+            const int sc = (b->n_slices - s0) < chunk ? (b->n_slices - s0) : chunk;
+            for (int e = 0; e < nE; ++e) {
+                float2* Xe = X + (size_t)e * sc * n1 * ldh;
+                const float* in = hist + ((int64_t)e * b->n_slices + s0) * n1 * n2;
+                int rc = line_dft(b->p2, false, kReal, in, rows_of(n2), (int64_t)sc * n1, Xe,
+                                  rows_of(ldh), H, false, 0.f, work, b->tw, s);
+                if (rc) return rc;
+                // columns of sc images (n1 rows of ldh) of this element
+                const LineMap cm{H, (int64_t)n1 * ldh, 1, ldh};
+                rc = line_dft(b->p1, false, kComplex, Xe, cm, (int64_t)sc * H, Xe, cm, n1,
+                              false, 0.f, work, b->tw, s);
+                if (rc) return rc;
+            }
+            const int64_t total = (int64_t)sc * n1 * H;
+            EMSIM_LAUNCH(0, k_half_spectrum_sum, dim3(grid_for(total, 256)), dim3(256), 0, s, X,
+                         b->sf, nE, sc, n1, H, ldh, n2, symmetric_bandlimit ? 1 : 0, r * r,
+                         inv_npix);
+            const LineMap cm{H, (int64_t)n1 * ldh, 1, ldh};
+            int rc = line_dft(b->p1, true, kComplex, X, cm, (int64_t)sc * H, X, cm, n1, false,
+                              0.f, work, b->tw, s);
+            if (rc) return rc;
+            rc = line_dft(b->p2, true, kHalf, X, rows_of(ldh), (int64_t)sc * n1,
+                          out + (int64_t)s0 * n1 * n2, rows_of(n2), n2, true, 1e-13f, work,
+                          b->tw, s);
+            if (rc) return rc;
+        }
+        return EMSIM_HIP_OK;
+    }
     for (int s0 = 0; s0 < b->n_slices; s0 += chunk) {
         const int sc = (b->n_slices - s0) < chunk ? (b->n_slices - s0) : chunk;
         int rc = r2c_pairs(b->n2, hist, X, (int64_t)b->n_elems * sc * (b->n1 / 2), b->n1, sc,
@@ -787,10 +1017,10 @@ extern "C" int emsim_hip_wave_propagator_create(emsim_hip_wave_propagator** out,
                                                 float relativity_gamma) {
     if (!out) return fail(EMSIM_HIP_EINVAL, "wave_propagator_create: null output handle");
     *out = nullptr;
-    if (!pow2_supported(n1) || !pow2_supported(n2))
+    if (!size_supported(n1) || !size_supported(n2))
         return fail(EMSIM_HIP_EUNSUPPORTED,
                     "wave_propagator_create: wave shape %dx%d; the hip backend implements "
-                    "powers of two from 8 to %d", n1, n2, kTwN);
+                    "sizes from 8 to %d (powers of two up to %d)", n1, n2, kChirpMaxN, kTwN);
     if (!(d1 > 0) || !(d2 > 0) || !(wave_length > 0))
         return fail(EMSIM_HIP_EINVAL, "wave_propagator_create: sizes must be positive");
     emsim_hip_wave_propagator* w = new (std::nothrow) emsim_hip_wave_propagator();
@@ -811,8 +1041,19 @@ extern "C" int emsim_hip_wave_propagator_create(emsim_hip_wave_propagator** out,
     w->filter2 = filt * filt;
     w->factor = wave_length * relativity_gamma;   // WavePropagator.cpp:42
     int rc = get_twiddles(&w->tw);
+    if (!rc && !(pow2_supported(n1) && pow2_supported(n2))) {
+        w->chirp = true;
+        rc = chirp_plan_create(n1, w->p1);
+        if (!rc) rc = chirp_plan_create(n2, w->p2);
+        if (!rc) {
+            const int64_t lines = std::max((int64_t)n1 * w->p2.M, (int64_t)n2 * w->p1.M);
+            if (hipMalloc(&w->work, sizeof(float2) * (size_t)lines) != hipSuccess)
+                rc = fail(EMSIM_HIP_ENOMEM, "wave_propagator_create: no device memory for "
+                                            "the chirp-z line buffer");
+        }
+    }
     if (rc) {
-        delete w;
+        emsim_hip_wave_propagator_destroy(w);
         return rc;
     }
     *out = w;
@@ -820,6 +1061,10 @@ extern "C" int emsim_hip_wave_propagator_create(emsim_hip_wave_propagator** out,
 }
 
 extern "C" int emsim_hip_wave_propagator_destroy(emsim_hip_wave_propagator* w) {
+    if (!w) return EMSIM_HIP_OK;
+    chirp_plan_free(w->p1);
+    chirp_plan_free(w->p2);
+    if (w->work) (void)hipFree(w->work);
     delete w;
     return EMSIM_HIP_OK;
 }
@@ -852,6 +1097,8 @@ extern "C" int emsim_hip_space_propagate(const emsim_hip_wave_propagator* w, con
     hipStream_t s = (hipStream_t)stream_;
     if (!w || !wave || !out || batch < 1) return fail(EMSIM_HIP_EINVAL, "space_propagate: bad arguments");
     if (batch > 65535) return fail(EMSIM_HIP_EUNSUPPORTED, "space_propagate: batch <= 65535");
+    if (w->chirp)
+        return chirp_filter(w, (const float2*)wave, (float2*)out, batch, make_propagator(w, dz), s);
     const int64_t rows = (int64_t)batch * w->n1;
     const int64_t npix = (int64_t)w->n1 * w->n2;
     float2* o = (float2*)out;
@@ -874,6 +1121,22 @@ extern "C" int emsim_hip_multislice_propagate(const emsim_hip_wave_propagator* w
     const int64_t npix = (int64_t)w->n1 * w->n2;
     const PropagatorMul prop = make_propagator(w, dz);
     float2* o = (float2*)out;
+    if (w->chirp) {
+        // image by image: transmission, then the propagator through chirp-z
+        // transforms (WavePropagator.cpp:90-103)
+        for (int b = 0; b < batch; ++b) {
+            float2* ob = o + b * npix;
+            for (int sl = 0; sl < n_slices; ++sl) {
+                const float2* src = sl == 0 ? (const float2*)wave + b * npix : ob;
+                EMSIM_LAUNCH(0, k_transmit, dim3(grid_for(npix, 256)), dim3(256), 0, s, src,
+                             slices + b * slices_batch_stride + (int64_t)sl * npix, ob, npix, npix,
+                             (int64_t)0, w->factor);
+                const int rc = chirp_filter(w, ob, ob, 1, prop, s);
+                if (rc) return rc;
+            }
+        }
+        return EMSIM_HIP_OK;
+    }
     for (int sl = 0; sl < n_slices; ++sl) {
         const float2* src = sl == 0 ? (const float2*)wave : o;
         int rc = row_transmit_fft(w->n2, sl == 0, src, slices + (int64_t)sl * npix, o, rows,
@@ -891,6 +1154,9 @@ extern "C" int emsim_hip_lens_propagate(const emsim_hip_wave_propagator* w, cons
     hipStream_t s = (hipStream_t)stream_;
     if (!w || !wave || !out || batch < 1) return fail(EMSIM_HIP_EINVAL, "lens_propagate: bad arguments");
     if (batch > 65535) return fail(EMSIM_HIP_EUNSUPPORTED, "lens_propagate: batch <= 65535");
+    if (w->chirp)
+        return chirp_filter(w, (const float2*)wave, (float2*)out, batch,
+                            make_lens(w, cs_mm, defocus, aperture), s);
     const int64_t rows = (int64_t)batch * w->n1;
     const int64_t npix = (int64_t)w->n1 * w->n2;
     float2* o = (float2*)out;
@@ -913,6 +1179,14 @@ extern "C" int emsim_hip_lens_intensity(const emsim_hip_wave_propagator* w, cons
     const int64_t rows = (int64_t)batch * w->n1;
     const int64_t npix = (int64_t)w->n1 * w->n2;
     float2* o = (float2*)scratch;
+    if (w->chirp) {
+        const int rc = chirp_filter(w, (const float2*)wave, o, batch,
+                                    make_lens(w, cs_mm, defocus, aperture), s);
+        if (rc) return rc;
+        EMSIM_LAUNCH(0, k_intensity, dim3(grid_for(npix * batch, 256)), dim3(256), 0, s,
+                     (const float2*)o, image, npix * batch);
+        return EMSIM_HIP_OK;
+    }
     int rc = row_fft(w->n2, false, (const float2*)wave, o, rows, w->tw, s);
     if (!rc)
         rc = col_lens(w->n1, o, w->n2, w->n2, npix, batch, make_lens(w, cs_mm, defocus, aperture),
@@ -1261,6 +1535,10 @@ int check_simulate(const emsim_hip_slice_builder* b, const emsim_hip_wave_propag
     if (w->n1 != b->n1 || w->n2 != b->n2)
         return fail(EMSIM_HIP_EINVAL, "simulate: builder grid %dx%d != propagator grid %dx%d",
                     b->n1, b->n2, w->n1, w->n2);
+    if (b->chirp || w->chirp)
+        return fail(EMSIM_HIP_EUNSUPPORTED,
+                    "simulate: the batched fused path needs power-of-two lateral sizes (%dx%d); "
+                    "other sizes run through the per-call API", b->n1, b->n2);
     if (symmetric_bandlimit && b->n1 != b->n2)
         return fail(EMSIM_HIP_EINVAL, "simulate: the symmetric band-limit needs a square grid");
     if (batch > 65535) return fail(EMSIM_HIP_EUNSUPPORTED, "simulate: batch <= 65535");

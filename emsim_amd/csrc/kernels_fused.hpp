@@ -259,6 +259,9 @@ __device__ __forceinline__ unsigned spec_item(unsigned L, unsigned total) {
 #define EMSIM_SPEC_MIN_WAVES 4
 #endif
 
+#ifndef EMSIM_ROW_UNROLL
+#define EMSIM_ROW_UNROLL 1
+#endif
 // Row DFTs of this thread's EPT rows r0 + m T at column col: the twiddles
 // w^(col y) of the row's atoms summed in float32, in bucket order (sorted).
 // rows holds absolute bucket positions (minus base: the position of
@@ -279,6 +282,7 @@ __device__ __forceinline__ void row_dft(float2 (&v)[EPT], bool valid, int r0, in
             const uint32_t beg = rows[r] - base, end = rows[r + 1] - base;
             if (!packed) {
                 uint32_t k = beg;
+#if EMSIM_ROW_UNROLL >= 2
                 // two atoms per step: both twiddle loads in flight together
                 for (; k + 2 <= end; k += 2) {
                     const int p0 = (int)__umul24((unsigned)col, bkt[k]) & mask;
@@ -290,7 +294,8 @@ __device__ __forceinline__ void row_dft(float2 (&v)[EPT], bool valid, int r0, in
                     sr += w1.x;
                     si += w1.y;
                 }
-                if (k < end) {
+#endif
+                for (; k < end; ++k) {
                     const float2 w = tws[((int)__umul24((unsigned)col, bkt[k]) & mask) * tw_step];
                     sr += w.x;
                     si += w.y;

@@ -8,8 +8,9 @@ error.
 ``emsim_hip_fno_*``): 4 spectral layers of 32 channels keeping 16 modes per
 axis by default, bf16 MFMA with float32 accumulation, forward, backward and
 Adam all in hand-written HIP kernels.  Its input is the exit wave of a
-coarse-slice (8 A) multislice run plus that run's projected potentials; its
-output is the correction towards the fine-slice (1 A) exit wave:
+coarse-slice multislice run (the shipped network: 4 slices of 25 A for a
+100 A thick box) plus that run's projected potentials; its output is the
+correction towards the fine-slice (1 A) exit wave:
 psi = psi_coarse + psi0 * (out_r + i out_i).
 
 ``CoarseFinePairs`` makes training pairs with the plain fused GPU path (the

@@ -38,7 +38,12 @@ typedef struct ihipStream_t* emsim_stream_t; /* == hipStream_t */
 /* ---- library ---------------------------------------------------------- */
 int emsim_hip_abi_version(void);
 const char* emsim_hip_last_error(void);
-/* 1 if n is a lateral size the FFT kernels implement (2^k, 8 <= n <= 4096). */
+/* Lateral sizes: 1 for a power of two from 8 to 4096 (native FFT kernels,
+ * every entry point), 2 for any other size from 8 to 2048 (chirp-z
+ * transforms on the power-of-two kernels: the per-call slice builder and
+ * wave propagator entry points; emsim_hip_simulate* refuse them), 0 else.
+ * The reference's FFTW plans take any size (fftw_ext/src/WavePropagator.cpp
+ * :25-33, SliceBuilder.cpp:41-55). */
 int emsim_hip_fft_size_supported(int n);
 
 /* Host-only helper, exported for the binning tests: the float64 bin edges

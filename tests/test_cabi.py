@@ -46,21 +46,23 @@ def test_library_exports_every_declared_symbol():
 def test_argument_errors_without_a_device():
     hip_ext = pytest.importorskip("emsim_amd.backend.hip_ext")
     h = ctypes.c_void_p()
-    rc = hip_ext.raw("emsim_hip_wave_propagator_create")(ctypes.byref(h), 100, 100, 1.0, 1.0,
+    rc = hip_ext.raw("emsim_hip_wave_propagator_create")(ctypes.byref(h), 3000, 100, 1.0, 1.0,
                                                          0.025, 1.39)
     assert rc == hip_ext.EUNSUPPORTED and not h.value
-    assert "powers of two" in hip_ext.last_error()
+    assert "sizes from 8 to 2048" in hip_ext.last_error()
     rc = hip_ext.raw("emsim_hip_wave_propagator_create")(ctypes.byref(h), 64, 64, -1.0, 1.0,
                                                          0.025, 1.39)
     assert rc == hip_ext.EINVAL
-    with pytest.raises(ValueError, match="powers of two"):
-        hip_ext.call("emsim_hip_wave_propagator_create", ctypes.byref(h), 96, 64, 1.0, 1.0,
+    with pytest.raises(ValueError, match="powers of two up to 4096"):
+        hip_ext.call("emsim_hip_wave_propagator_create", ctypes.byref(h), 4, 64, 1.0, 1.0,
                      0.025, 1.39)
     rc = hip_ext.raw("emsim_hip_slice_builder_create")(ctypes.byref(h), None, 1, 1, 64, 64, 1.0,
                                                        1.0, 1.0, 0)
     assert rc == hip_ext.EINVAL
     assert hip_ext.raw("emsim_hip_intensity")(None, None, 10, None) == hip_ext.EINVAL
-    assert hip_ext.fft_size_supported(512) and not hip_ext.fft_size_supported(500)
+    assert hip_ext.fft_size_supported(512) and hip_ext.fft_size_supported(500)
+    assert not hip_ext.fft_size_supported(5000) and not hip_ext.fft_size_supported(7)
+    assert hip_ext.raw("emsim_hip_fft_size_supported")(500) == 2
     assert hip_ext.abi_version() == 2
     # learned correction: shapes are checked before any device allocation
     f = ctypes.c_void_p()

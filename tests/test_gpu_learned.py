@@ -28,9 +28,10 @@ def setup(hip):
                     "emsim_amd/data/fno_512.npz)")
     from emsim_amd.learned import FNO, CoarseFinePairs
     fno = FNO.load(str(CKPT))
-    pairs = CoarseFinePairs(_tem(), n=fno.n, fine_slices=100, coarse_dz=8.0,
+    coarse_dz = fno.coarse_dz if fno.coarse_dz is not None else 8.0
+    pairs = CoarseFinePairs(_tem(), n=fno.n, fine_slices=100, coarse_dz=coarse_dz,
                             coarse_slices=fno.cin - 2)
-    # held out: seeds far from the training seeds (100 + ...)
+    # held out: seeds far from the training pool's (101 ...)
     sizes = [12000, 25000, 40000, 70000]
     quats = random_uniform_quaternions(len(sizes), rng=np.random.default_rng(20001))
     mols = [synth.rotated(synth.protein_cloud(s, 20000 + i), quats[i]) for i, s in enumerate(sizes)]
@@ -46,7 +47,7 @@ def test_network_gain_over_coarse_slicing(setup):
     pred = fno.apply(cw, fno.forward(x), pairs.psi0)
     learned = wave_loss(pred, fine)[0][:, 0].sqrt().cpu().numpy()
     coarse = wave_loss(cw, fine)[0][:, 0].sqrt().cpu().numpy()
-    print(f"exit-wave rel L2 vs plain 1 A multislice: coarse 8 A alone {coarse.round(5)}, "
+    print(f"exit-wave rel L2 vs plain 1 A multislice: coarse slices alone {coarse.round(5)}, "
           f"with the network {learned.round(5)}")
     assert learned.max() < 0.01
     assert learned.mean() < 0.5 * coarse.mean()
