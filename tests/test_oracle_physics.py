@@ -85,3 +85,48 @@ def test_half_table_oracle_equals_package(n1, n2, d1, d2):
     assert np.all(a[:, 0, 0] == a.reshape(5, -1).max(1))
     # symmetric in +-k1 (real, even form factors)
     np.testing.assert_array_equal(a[:, 1:, :], a[:, :0:-1, :])
+
+
+def test_form_factor_is_the_hankel_transform_of_the_projected_potential():
+    """The oracle's f_e(q) table against the real-space projected potential
+    of the same Kirkland fit (emsim/elem.py:67-117, Kirkland eq. 5.10):
+        v_z(r) = 2 pi a0 e [2 pi sum a_i K0(2 pi r sqrt b_i)
+                            + pi sum c_i / d_i exp(-pi^2 r^2 / d_i)],
+    whose 2-D Fourier transform, 2 pi int v_z(r) J0(2 pi q r) r dr, is
+    2 pi a0 e f_e(q).  The transform is integrated numerically, so a swapped
+    or misread parameter column, a dropped term or a wrong q scale in the
+    table fails here."""
+    special = pytest.importorskip("scipy.special")
+    integrate = pytest.importorskip("scipy.integrate")
+    zs = [1, 6, 8, 16, 79]
+    n, d = 64, 0.25                            # table frequencies j / (n d) = j / 16
+    sf = orc.scattering_factors_half(zs, n, n, d, d)
+    p = orc.kirkland_params(zs).astype(np.float64)
+    edges = np.linspace(0.0, 60.0, 241)
+    for k in range(len(zs)):
+        a, b, c, dd = p[k, 1:4], p[k, 4:7], p[k, 7:10], p[k, 10:13]
+
+        def v(r):
+            return (2 * math.pi * sum(a[i] * special.k0(2 * math.pi * r * math.sqrt(b[i]))
+                                      for i in range(3))
+                    + math.pi * sum(c[i] / dd[i] * math.exp(-math.pi ** 2 * r * r / dd[i])
+                                    for i in range(3)))
+        for j in (0, 2, 8, 16, 32):
+            q = j / (n * d)
+            f = lambda r: 2 * math.pi * v(r) * special.j0(2 * math.pi * q * r) * r
+            val = sum(integrate.quad(f, lo, hi, limit=200)[0]
+                      for lo, hi in zip(edges[:-1], edges[1:]))
+            assert abs(val - float(sf[k, 0, j])) <= 1e-6 * val, (zs[k], q)
+            assert sf[k, j, 0] == sf[k, 0, j]      # radial: the same q along either axis
+
+
+def test_form_factor_lorentzian_weights_follow_the_mott_asymptote():
+    """At large q the Mott-Bethe formula gives f_e(q) -> Z / (2 pi^2 a0 q^2),
+    so the Lorentzian weights of the fit sum to about Z / (2 pi^2 a0) (a0 in
+    Angstrom); Kirkland's fits meet it to a few per cent."""
+    zs = [1, 6, 7, 8, 16, 29, 79]
+    p = orc.kirkland_params(zs).astype(np.float64)
+    a0_angstrom = physics.a0 * 1e8
+    for z, row in zip(zs, p):
+        ratio = row[1:4].sum() / (z / (2 * math.pi ** 2 * a0_angstrom))
+        assert abs(ratio - 1) < 0.04, (z, ratio)
