@@ -69,6 +69,8 @@ def parse():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--learned-only", action="store_true",
                     help="run only the learned coarse-slice line (prints its JSON object)")
+    ap.add_argument("--config-only", choices=["C2", "C3", "C4"], default=None,
+                    help="run only one configuration line (prints its JSON object)")
     return ap.parse_args()
 
 
@@ -360,17 +362,27 @@ def _wall(fn, reps):
     return (time.perf_counter() - t0) / reps
 
 
-def config_lines(tem):
+def config_lines(tem, which=("C2", "C4", "C3")):
     """C2 (one 50k-atom cloud in 64 orientations rotated on the device), C3
     (the 2048^2 x 400-slice slab, one image), C4 (256 molecules of 5k-200k
     atoms on 128^2/256^2/512^2 waves, from host AtomLists as they come):
     images/s and the fraction of the wall time the GPU spends in kernels."""
+    out = {}
+    if "C2" in which:
+        out.update(_config_c2(tem))
+    if "C4" in which:
+        out.update(_config_c4(tem))
+    if "C3" in which:
+        out.update(_config_c3(tem))
+    return out
+
+
+def _config_c2(tem):
     from emsim_amd import atoms as atm
     from emsim_amd import synth
-    from emsim_amd.batch import BatchPipe, MixedBatch
+    from emsim_amd.batch import BatchPipe
     from emsim_amd.utils.rot import random_uniform_quaternions
     out = {}
-    # C2
     mol = synth.protein_cloud(50000, 0)
     ns2 = int(np.ceil(float(atm.centralize(mol).space[0]) / 2.0))
     p2 = BatchPipe(tem, 1.0, 2.0, 256, ns2, [1, 6, 7, 8, 16])
@@ -384,8 +396,13 @@ def config_lines(tem):
                  "images_per_s": round(64 / t, 1), "ms_per_image_batch1": round(t1 * 1e3, 3),
                  "gpu_busy_frac": round(k * 1e-3 / t, 3)}
     progress(f"C2: {out['C2']['images_per_s']} images/s")
-    del b2, one, p2
-    # C4
+    return out
+
+
+def _config_c4(tem):
+    from emsim_amd import synth
+    from emsim_amd.batch import MixedBatch
+    out = {}
     t0 = time.perf_counter()
     items = synth.mixed_workload(count=256, seed=2, lo=5000, hi=200000)
     gen = time.perf_counter() - t0
@@ -403,8 +420,13 @@ def config_lines(tem):
                  "host_prepared_images_per_s": round(256 / th, 1),
                  "generation_s": round(gen, 1)}
     progress(f"C4: {out['C4']['images_per_s']} images/s")
-    del items, mb
-    # C3
+    return out
+
+
+def _config_c3(tem):
+    from emsim_amd import synth
+    from emsim_amd.batch import BatchPipe
+    out = {}
     slab = synth.large_slab(seed=1)
     p3 = BatchPipe(tem, 0.5, 1.0, 2048, 400, [1, 6, 7, 8, 16])
     b3 = p3.upload([p3.prepare(slab)])
@@ -623,6 +645,9 @@ def main():
                elements=[1, 6, 7, 8, 16], n_elems=5)
     tem = emsim_amd.tem.TEM(electron_dose=20, beam_energy_kev=200, cs_mm=1.3, defocus=700,
                             aperture=10e-3)
+    if args.config_only:
+        print(json.dumps(config_lines(tem, (args.config_only,))), flush=True)
+        return
     mols, sizes = workload(args.batch)
     progress(f"workload: {args.batch} molecules, mean {np.mean(sizes):.0f} atoms")
     if args.learned_only:

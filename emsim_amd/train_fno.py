@@ -1,6 +1,6 @@
 """Train the learned coarse-slice correction on pairs the plain GPU path makes.
 
-    python -m emsim_amd.train_fno --steps 2500 --batch 8 --coarse-dz 16 --out fno_512.npz
+    python -m emsim_amd.train_fno --steps 2500 --batch 8 --coarse-dz 25 --out fno_512.npz
 
 Every step draws a batch of synthetic protein-like clouds (atom counts
 log-uniform in 10k-100k) from a pool of base clouds (seeds 101 and up) in
@@ -69,7 +69,7 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--slices", type=int, default=100)
-    ap.add_argument("--coarse-dz", type=float, default=16.0)
+    ap.add_argument("--coarse-dz", type=float, default=25.0)
     ap.add_argument("--channels", type=int, default=32)
     ap.add_argument("--modes", type=int, default=16)
     ap.add_argument("--layers", type=int, default=4)

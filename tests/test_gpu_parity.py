@@ -250,11 +250,13 @@ def test_bad_arguments_raise():
         p.space_propagate(np.zeros((32, 32), np.complex64), 1.0)
     with pytest.raises(TypeError):
         p.space_propagate(np.zeros((64, 64), np.float32), 1.0)
-    with pytest.raises(ValueError):
-        emsim_amd.wave.get_raw_wave_propagator(100, 1.0, 200)
+    # sizes outside 8..4096 (powers of two) and 8..2048 (chirp-z)
+    for n in (4, 3000, 8192):
+        with pytest.raises(ValueError):
+            emsim_amd.wave.get_raw_wave_propagator(n, 1.0, 200)
     from emsim_amd.backend.slice_builder_hip import MultiSlicesBuilder
     with pytest.raises(ValueError):
-        MultiSlicesBuilder([6], 3, 96, 96, 1.0, 1.0, 1.0)
+        MultiSlicesBuilder([6], 3, 3000, 3000, 1.0, 1.0, 1.0)
     # numpy.arange makes 9 edges for 7 slices of 0.3 A: the reference would
     # build an 8-slice histogram and misalign it; the hip builder refuses
     with pytest.raises(ValueError, match="edges"):

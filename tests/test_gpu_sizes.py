@@ -16,7 +16,7 @@ from emsim_amd import atoms as atm
 from emsim_amd import pot
 from emsim_amd import synth
 from oracle import emsim_oracle as orc
-from tests._chain import pipe_oracle, slices_oracle
+from tests._chain import slices_oracle
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-5
@@ -92,20 +92,40 @@ def test_one_slice_odd_size():
     assert e < TOL
 
 
-def test_pipe_sizes_grid_from_the_molecule():
-    """Pipe with roi=None: the lateral size follows the molecule's extent
-    (ceil(extent / pixel), not a power of two), as the reference's Pipe does."""
+def test_grid_sized_from_the_molecule():
+    """pot.build_multi_slices without lateral_size sizes the grid from the
+    molecule (ceil(extent / pixel) per axis, not a power of two); the slices,
+    the propagator built for their shape, multislice, lens and |psi|^2 match
+    the oracle chain.  (The reference's Pipe with roi=None fails with a
+    TypeError before any of this, wave.py:10-16 indexing None; so does this
+    Pipe.)"""
     tem = emsim_amd.tem.TEM(electron_dose=20, beam_energy_kev=200, cs_mm=1.3, defocus=700,
                             aperture=10e-3)
+    cloud = synth.protein_cloud(6000, 410)
+    # two H atoms at opposite corners square the x-y extent (49.5 A), so the
+    # reference's band-limit, which needs n1 == n2, applies
+    corners = np.array([[0.0, -24.75, -24.75], [0.0, 24.75, 24.75]], dtype=np.float32)
+    mol = atm.centralize(atm.AtomList(np.concatenate([cloud.elements, [1, 1]]),
+                                      np.concatenate([cloud.coordinates, corners])))
+    v = pot.build_multi_slices(mol, dz=2.0, pixel_size=1.0)
+    ns, n1, n2 = tuple(v.shape)
+    assert n1 & (n1 - 1) and n2 & (n2 - 1)
+    prop = emsim_amd.wave.get_raw_wave_propagator((n1, n2), 1.0, 200)
+    exit_wave = prop.multislice_propagate(prop.init_wave(tem.electron_dose), v, 2.0)
+    image = _host(prop.lens_intensity(exit_wave, tem.cs_mm, tem.defocus, tem.aperture))
+    v32, v64, _ = slices_oracle(mol, 2.0, 1.0, None)
+    assert v32.shape == (ns, n1, n2)
+    w0 = orc.init_wave(n1, n2, 1.0, 1.0, tem.electron_dose)
+    x32 = orc.multislice_fftw32(w0, v32.astype(np.float32), 2.0, LAM, GAMMA, 1.0, 1.0)
+    i32 = orc.intensity(orc.lens_propagate_fftw32(x32, LAM, tem.cs_mm, tem.defocus,
+                                                  tem.aperture, 1.0, 1.0))
+    e = orc.rel_l2(image.reshape(i32.shape), i32)
+    print(f"[grid from the molecule {n1}x{n2}, {ns} slices] image vs fftw32 {e:.2e}", flush=True)
+    assert e < TOL
     pipe = emsim_amd.pipe.Pipe(tem, resolution=2.0, slice_thickness=2.0, roi=None,
                                add_water=False)
-    mol = synth.protein_cloud(6000, 410)
-    image = _host(pipe.run(mol))
-    i32, i64, _, _ = pipe_oracle(mol, tem, 2.0, 2.0, None)
-    assert image.shape == i32.shape and image.shape[0] & (image.shape[0] - 1)
-    e32, e64 = orc.rel_l2(image, i32), orc.rel_l2(image, i64)
-    print(f"[Pipe roi=None {image.shape}] image vs fftw32 {e32:.2e}, fp64 {e64:.2e}", flush=True)
-    assert e32 < TOL and e64 < TOL
+    with pytest.raises(TypeError):
+        pipe.run(mol)
 
 
 def test_batched_waves_equal_single_and_fused_path_refuses():
