@@ -259,6 +259,27 @@ __device__ __forceinline__ unsigned spec_item(unsigned L, unsigned total) {
 #define EMSIM_SPEC_MIN_WAVES 4
 #endif
 
+// Copy src[0, count) to LDS: in each round a thread issues all PER of its
+// loads before its stores, from indices clamped into range, so no lane's
+// load waits behind a branch or behind the previous entry's store.
+template <int PER, typename T>
+__device__ __forceinline__ void stage_lds(T* dst, const T* __restrict__ src, uint32_t count,
+                                          int tid, int nthreads) {
+    for (uint32_t base = 0; base < count; base += (uint32_t)(PER * nthreads)) {
+        T v[PER];
+#pragma unroll
+        for (int r = 0; r < PER; ++r) {
+            const uint32_t i = base + (uint32_t)(tid + r * nthreads);
+            v[r] = src[i < count ? i : count - 1];
+        }
+#pragma unroll
+        for (int r = 0; r < PER; ++r) {
+            const uint32_t i = base + (uint32_t)(tid + r * nthreads);
+            if (i < count) dst[i] = v[r];
+        }
+    }
+}
+
 #ifndef EMSIM_ROW_UNROLL
 #define EMSIM_ROW_UNROLL 1
 #endif
@@ -385,7 +406,7 @@ __device__ __forceinline__ void row_dft_big(float2 (&v)[EPT], bool valid, int r0
     for (uint32_t p0 = e_beg; p0 < e_end; p0 += (uint32_t)CAP) {
         const uint32_t p1 = e_end - p0 > (uint32_t)CAP ? p0 + CAP : e_end;
         __syncthreads();   // the previous piece is consumed
-        for (uint32_t i = threadIdx.x; i < p1 - p0; i += block) lds_bkt[i] = bucket[p0 + i];
+        stage_lds<16>(lds_bkt, bucket + p0, p1 - p0, threadIdx.x, block);
         __syncthreads();
         if (!valid) continue;
 #pragma unroll
@@ -434,27 +455,6 @@ __device__ __forceinline__ void spec_fft(float2 (&v)[EPT], float* re, float* im,
                                          const float2* lds_tw, const float2* __restrict__ tw) {
     if constexpr (LDS) fft_n<N, EPT, INV>(v, re, im, t, TwLds<N>{lds_tw});
     else fft_n<N, EPT, INV>(v, re, im, t, tw);
-}
-
-// Copy src[0, count) to LDS: in each round a thread issues all PER of its
-// loads before its stores, from indices clamped into range, so no lane's
-// load waits behind a branch or behind the previous entry's store.
-template <int PER, typename T>
-__device__ __forceinline__ void stage_lds(T* dst, const T* __restrict__ src, uint32_t count,
-                                          int tid, int nthreads) {
-    for (uint32_t base = 0; base < count; base += (uint32_t)(PER * nthreads)) {
-        T v[PER];
-#pragma unroll
-        for (int r = 0; r < PER; ++r) {
-            const uint32_t i = base + (uint32_t)(tid + r * nthreads);
-            v[r] = src[i < count ? i : count - 1];
-        }
-#pragma unroll
-        for (int r = 0; r < PER; ++r) {
-            const uint32_t i = base + (uint32_t)(tid + r * nthreads);
-            if (i < count) dst[i] = v[r];
-        }
-    }
 }
 
 template <int N>
