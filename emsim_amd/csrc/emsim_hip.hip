@@ -1780,8 +1780,8 @@ int simulate_core(const emsim_hip_slice_builder* b, const emsim_hip_wave_propaga
         EMSIM_LAUNCH(0, k_csr_scatter, dim3(grid_for(n_atoms, 256)), dim3(256), 0, s, n_atoms,
                      ws.atom_row, ws.atom_y, ws.row_start, ws.row_count, ws.bucket);
         // a fixed order of every row's atoms for the float row sums
-        EMSIM_LAUNCH(0, k_csr_sort_rows, dim3(grid_for(ws.n_rows, 256)), dim3(256), 0, s,
-                     ws.n_rows, ws.row_start, ws.bucket);
+        EMSIM_LAUNCH(0, k_csr_sort_rows, dim3((unsigned)ceil_div(ws.n_rows, kSortRows)),
+                     dim3(kSortRows), 0, s, ws.n_rows, ws.row_start, ws.bucket);
     }
     // per-slice molecule order of the spectrum pass (identity beyond one
     // block's worth of molecules)

@@ -111,32 +111,55 @@ __global__ void k_csr_scatter(int64_t n_atoms, const int32_t* __restrict__ atom_
     }
 }
 
-// Sort every bucket row's y bins ascending (one thread per row; rows hold a
-// few atoms, dense ones tens, so insertion sort, and a gapped pre-pass for
-// the rare long row), which fixes the order of the float row sums.
-__global__ void k_csr_sort_rows(int64_t n_rows, const uint32_t* __restrict__ row_start,
-                                uint16_t* __restrict__ bucket) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n_rows; r += stride) {
-        const uint32_t beg = row_start[r], len = row_start[r + 1] - beg;
-        if (len < 2) continue;
-        uint16_t* a = bucket + beg;
-        // Shell-sort gaps (Ciura) for long rows; gap 1 is the insertion sort
-        const uint32_t gaps[] = {701u, 301u, 132u, 57u, 23u, 10u, 4u, 1u};
-        for (uint32_t gi = 0; gi < 8; ++gi) {
-            const uint32_t g = gaps[gi];
-            if (g >= len) continue;
-            for (uint32_t i = g; i < len; ++i) {
-                const uint16_t v = a[i];
-                uint32_t j = i;
-                while (j >= g && a[j - g] > v) {
-                    a[j] = a[j - g];
-                    j -= g;
-                }
-                a[j] = v;
+// Sort a[0, len) ascending: insertion sort, after a gapped (Shell, Ciura's
+// gaps) pre-pass for the rare long row.
+__device__ __forceinline__ void sort_row(uint16_t* a, uint32_t len) {
+    const uint32_t gaps[] = {701u, 301u, 132u, 57u, 23u, 10u, 4u, 1u};
+    for (uint32_t gi = 0; gi < 8; ++gi) {
+        const uint32_t g = gaps[gi];
+        if (g >= len) continue;
+        for (uint32_t i = g; i < len; ++i) {
+            const uint16_t v = a[i];
+            uint32_t j = i;
+            while (j >= g && a[j - g] > v) {
+                a[j] = a[j - g];
+                j -= g;
             }
+            a[j] = v;
         }
     }
+}
+
+// Sort every bucket row's y bins ascending, which fixes the order of the
+// float row sums.  One block per kSortRows consecutive rows, one thread per
+// row: their bucket range (contiguous) is staged in LDS when it fits, each
+// thread sorts its row there, and the range is written back, so the sort's
+// dependent accesses are LDS round trips; a larger range is sorted in place
+// in global memory.
+constexpr int kSortRows = 256;
+constexpr uint32_t kSortLds = 16384;
+__global__ __launch_bounds__(kSortRows) void k_csr_sort_rows(int64_t n_rows,
+                                                             const uint32_t* __restrict__ row_start,
+                                                             uint16_t* __restrict__ bucket) {
+    __shared__ uint16_t lds[kSortLds];
+    const int64_t r0 = (int64_t)blockIdx.x * kSortRows;
+    const int64_t r1 = r0 + kSortRows < n_rows ? r0 + kSortRows : n_rows;
+    const int64_t r = r0 + threadIdx.x;
+    const uint32_t lo = row_start[r0], hi = row_start[r1];
+    uint32_t beg = 0, len = 0;
+    if (r < r1) {
+        beg = row_start[r];
+        len = row_start[r + 1] - beg;
+    }
+    if (hi - lo > kSortLds) {
+        if (len >= 2) sort_row(bucket + beg, len);
+        return;
+    }
+    for (uint32_t i = threadIdx.x; i < hi - lo; i += kSortRows) lds[i] = bucket[lo + i];
+    __syncthreads();
+    if (len >= 2) sort_row(lds + (beg - lo), len);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < hi - lo; i += kSortRows) bucket[lo + i] = lds[i];
 }
 
 // Dense count planes of one slice of every molecule, with the water fill.
