@@ -67,6 +67,10 @@ def summarize(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, copy_mib, li
 
 
 SQ_COUNTERS = ("SQ_INSTS_VALU", "SQ_INSTS_LDS", "SQ_INSTS_SALU", "SQ_WAVES")
+# also kept when the run collected them: cycles the LDS stalled on bank
+# conflicts, wave-cycles waiting to issue LDS instructions, wave-cycles
+# resident, and busy cycles
+SQ_OPTIONAL = ("SQ_LDS_BANK_CONFLICT", "SQ_WAIT_INST_LDS", "SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES")
 
 # VALU issue peak of an MI355X in wave64 instructions per second: 256 CUs x 4
 # SIMDs, one wave64 VALU instruction per SIMD every 4 cycles at 2.4 GHz
@@ -77,7 +81,7 @@ VALU_ISSUE_PEAK = 1024 * 2.4e9 / 4
 def summarize_sq(sq_csv, lib=None):
     """Per-kernel mean of each SQ instruction counter per dispatch."""
     kernels = {}
-    for counter in SQ_COUNTERS:
+    for counter in SQ_COUNTERS + SQ_OPTIONAL:
         for name, (mean, count) in _per_kernel(sq_csv, counter).items():
             rec = kernels.setdefault(name, {"dispatches": count})
             rec[counter] = mean
