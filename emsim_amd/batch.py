@@ -238,8 +238,10 @@ class BatchPipe:
         return lut[z]
 
     def pack_raw(self, mols: Sequence[atm.AtomList]):
-        """Host side of ``upload_raw``: (coordinates (n, 3) float32 as given,
-        element indices (n,) uint8, molecule offsets, labels)."""
+        """A raw batch as plain host arrays -- (coordinates (n, 3) float32 as
+        given, element indices (n,) uint8, molecule offsets, labels) -- for
+        a producer thread that packs while the GPU simulates (``BatchSim``);
+        ``upload_packed`` copies them to the GPU."""
         if len(mols) == 0:
             raise ValueError("empty batch")
         if len(self.elements) > 254:
@@ -261,8 +263,50 @@ class BatchPipe:
     def upload_raw(self, mols: Sequence[atm.AtomList]) -> RawBatch:
         """Molecules as they come, copied to the GPU once: coordinates as
         given (not centred, atoms in any order) and per-atom element indices;
-        ``run_raw`` prepares them on the device."""
-        return self.upload_packed(self.pack_raw(mols))
+        ``run_raw`` prepares them on the device.
+
+        The host writes both arrays straight into pinned staging buffers
+        (one slice per molecule, on a thread pool) and queues their copies
+        on the current stream without waiting for them, so the packing of
+        the next batch overlaps this one's simulation.  The buffers belong
+        to this pipe; the next call waits for these copies before refilling
+        them."""
+        if len(mols) == 0:
+            raise ValueError("empty batch")
+        if len(self.elements) > 254:
+            raise ValueError("at most 254 elements per pipe")
+        sizes = np.array([len(m) for m in mols], dtype=np.int64)
+        mol_off = np.zeros(len(mols) + 1, dtype=np.int64)
+        np.cumsum(sizes, out=mol_off[1:])
+        n = int(mol_off[-1])
+        st = self._staging(n)
+        ch, eh = st["coords"].numpy(), st["elem"].numpy()
+
+        def fill(i):
+            a, b = int(mol_off[i]), int(mol_off[i + 1])
+            eh[a:b] = self.element_indices(mols[i])
+            ch[a:b] = np.asarray(mols[i].coordinates, dtype=np.float32).reshape(-1, 3)
+
+        pool_map(fill, range(len(mols)))
+        dev = device()
+        coords = st["coords"][:n].to(dev, non_blocking=True)
+        elem = st["elem"][:n].to(dev, non_blocking=True)
+        st["done"].record()
+        return RawBatch(coords, elem, mol_off, [getattr(m, "label", "") for m in mols])
+
+    def _staging(self, n_atoms: int):
+        """Pinned host buffers of upload_raw for n_atoms atoms, once the
+        copies queued from them by the previous call are done."""
+        st = getattr(self, "_stage", None)
+        if st is not None:
+            st["done"].synchronize()
+        if st is None or st["coords"].shape[0] < n_atoms:
+            cap = max(int(n_atoms), 1)
+            st = self._stage = {
+                "coords": torch.empty((cap, 3), dtype=torch.float32, pin_memory=True),
+                "elem": torch.empty((cap,), dtype=torch.uint8, pin_memory=True),
+                "done": torch.cuda.Event()}
+        return st
 
     def upload_orientations(self, mol: atm.AtomList, quats: np.ndarray) -> PreparedBatch:
         """One molecule in len(quats) orientations, rotated and centred on the
