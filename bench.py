@@ -313,7 +313,7 @@ def learned_bench(args, cfg, tem, mols, plain_ips):
 # ---------------------------------------------------------------------------
 # host-to-host line
 # ---------------------------------------------------------------------------
-def end_to_end(pipe, mols, batch: int, repeats: int = 2):
+def end_to_end(pipe, mols, batch: int, repeats: int = 6):
     """BatchSim (the batched EMSim loop) from host AtomLists to host images:
     centring and sort by element on a host thread pool, one upload per batch,
     the fused GPU path, and a D2H copy into pinned buffers overlapped with the

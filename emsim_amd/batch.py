@@ -237,11 +237,12 @@ class BatchPipe:
                              f"list {self.elements}")
         return lut[z]
 
-    def pack_raw(self, mols: Sequence[atm.AtomList]):
+    def pack_raw(self, mols: Sequence[atm.AtomList], elem=None):
         """A raw batch as plain host arrays -- (coordinates (n, 3) float32 as
         given, element indices (n,) uint8, molecule offsets, labels) -- for
         a producer thread that packs while the GPU simulates (``BatchSim``);
-        ``upload_packed`` copies them to the GPU."""
+        ``upload_packed`` copies them to the GPU.  ``elem``: the molecules'
+        element indices when the caller has them already."""
         if len(mols) == 0:
             raise ValueError("empty batch")
         if len(self.elements) > 254:
@@ -249,16 +250,26 @@ class BatchPipe:
         sizes = np.array([len(m) for m in mols], dtype=np.int64)
         mol_off = np.zeros(len(mols) + 1, dtype=np.int64)
         np.cumsum(sizes, out=mol_off[1:])
-        elem = pool_map(self.element_indices, mols)
+        if elem is None:
+            elem = pool_map(self.element_indices, mols)
         coords = np.concatenate([np.asarray(m.coordinates, dtype=np.float32).reshape(-1, 3)
                                  for m in mols]) if mol_off[-1] else np.zeros((0, 3), np.float32)
         elem = np.concatenate(elem) if mol_off[-1] else np.zeros(0, np.uint8)
         return coords, np.ascontiguousarray(elem), mol_off, [getattr(m, "label", "") for m in mols]
 
     def upload_packed(self, packed) -> RawBatch:
+        """Copy a ``pack_raw`` batch to the GPU through this pipe's pinned
+        staging buffers, without waiting for the copy."""
         coords, elem, mol_off, labels = packed
-        return RawBatch(as_device(coords, torch.float32, "coordinates"),
-                        torch.from_numpy(elem).to(device()), mol_off, list(labels))
+        n = int(mol_off[-1])
+        st = self._staging(n)
+        np.copyto(st["coords"].numpy()[:n], np.asarray(coords, dtype=np.float32).reshape(-1, 3))
+        np.copyto(st["elem"].numpy()[:n], np.asarray(elem, dtype=np.uint8))
+        dev = device()
+        c = st["coords"][:n].to(dev, non_blocking=True)
+        e = st["elem"][:n].to(dev, non_blocking=True)
+        st["done"].record()
+        return RawBatch(c, e, mol_off, list(labels))
 
     def upload_raw(self, mols: Sequence[atm.AtomList]) -> RawBatch:
         """Molecules as they come, copied to the GPU once: coordinates as
@@ -609,9 +620,8 @@ class BatchSim:
         try:
             def attempt(mol):
                 try:
-                    if self.device_prep:
-                        self.pipe.element_indices(mol)   # refuses unknown elements
-                        return mol
+                    if self.device_prep:   # refuses unknown elements
+                        return mol, self.pipe.element_indices(mol)
                     return self.pipe.prepare(mol)
                 except Exception as e:   # reference: log the failed image, go on
                     return e
@@ -624,8 +634,11 @@ class BatchSim:
                         self.failed += 1
                     else:
                         prepared.append(res)
-                if prepared:
-                    q.put(self.pipe.pack_raw(prepared) if self.device_prep else prepared)
+                if prepared and self.device_prep:
+                    q.put(self.pipe.pack_raw([m for m, _ in prepared],
+                                             [e for _, e in prepared]))
+                elif prepared:
+                    q.put(prepared)
                 if stop.is_set():
                     break
         except Exception as e:   # an iterator that raises ends the stream
