@@ -282,6 +282,9 @@ struct emsim_hip_slice_builder {
     double dz = 0, d1 = 0, d2 = 0;
     int one_slice = 0;
     float* sf = nullptr;        // (n_elems, n1, H)
+    // the same, tile-major for the fused spectrum pass: (n_elems, tiles, n1,
+    // C) with C = spec_tile_cols(n1) columns per tile, zeros beyond H
+    float* sf_tiles = nullptr;
     float* edges = nullptr;     // axis0 (n0+1) | axis1 (n1+1) | axis2 (n2+1)
     int last_exact[3] = {0, 0, 0};
     float inv_d[3] = {0, 0, 0};
@@ -783,6 +786,18 @@ extern "C" int emsim_hip_slice_builder_create(emsim_hip_slice_builder** out,
     const size_t sf_bytes = (size_t)n_elems * n1 * b->H * sizeof(float);
     hipError_t err = hipMalloc(&b->sf, sf_bytes);
     if (err == hipSuccess) err = hipMemcpy(b->sf, sf_host, sf_bytes, hipMemcpyHostToDevice);
+    if (err == hipSuccess && pow2_supported(n1)) {
+        const int C = spec_tile_cols(n1), tiles = (int)ceil_div(b->H, C);
+        std::vector<float> t((size_t)n_elems * tiles * n1 * C, 0.f);
+        for (int e = 0; e < n_elems; ++e)
+            for (int x = 0; x < n1; ++x)
+                for (int j = 0; j < b->H; ++j)
+                    t[(((size_t)e * tiles + j / C) * n1 + x) * C + j % C] =
+                        sf_host[((size_t)e * n1 + x) * b->H + j];
+        err = hipMalloc(&b->sf_tiles, t.size() * sizeof(float));
+        if (err == hipSuccess)
+            err = hipMemcpy(b->sf_tiles, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice);
+    }
     if (err == hipSuccess) err = hipMalloc(&b->edges, edges32.size() * sizeof(float));
     if (err == hipSuccess)
         err = hipMemcpy(b->edges, edges32.data(), edges32.size() * sizeof(float),
@@ -800,6 +815,7 @@ extern "C" int emsim_hip_slice_builder_destroy(emsim_hip_slice_builder* b) {
     chirp_plan_free(b->p1);
     chirp_plan_free(b->p2);
     if (b->sf) (void)hipFree(b->sf);
+    if (b->sf_tiles) (void)hipFree(b->sf_tiles);
     if (b->edges) (void)hipFree(b->edges);
     delete b;
     return EMSIM_HIP_OK;
@@ -1386,7 +1402,8 @@ extern "C" int emsim_hip_timing_collect(char* buf, size_t cap, size_t* needed) {
 namespace {
 
 template <int N>
-int launch_spec_sparse(const uint32_t* row_start, const uint16_t* bucket, const float* sf, float2* Y, int slice, int n_slices, int n_elems, int H,
+int launch_spec_sparse(const uint32_t* row_start, const uint16_t* bucket, const float* sf,
+                       const float* sft, float2* Y, int slice, int n_slices, int n_elems, int H,
                        int ldh, int n2, int bandlimit, int band_r2, float inv_npix, int batch,
                        const int* order, const float2* tw, hipStream_t s) {
     using Cfg = SpecCfg<N>;
@@ -1395,9 +1412,10 @@ int launch_spec_sparse(const uint32_t* row_start, const uint16_t* bucket, const 
     // with spec_pack(N), column n2/2 is carried by column 0
     const int cols = spec_pack(N) ? n2 / 2 : H;
     const dim3 grid((unsigned)(ceil_div(cols, Cfg::C) * (int64_t)batch));
+    if (!sft) return fail(EMSIM_HIP_EINVAL, "spectrum: no tile-major form-factor table");
     EMSIM_LAUNCH(N, (k_spec_sparse<N>), grid, dim3(Cfg::BLOCK), 0, s, row_start, bucket, sf,
-                 Y, slice, n_slices, n_elems, H, ldh, n2, batch, bandlimit, band_r2, inv_npix,
-                 order, tw);
+                 sft, Y, slice, n_slices, n_elems, H, ldh, n2, batch, bandlimit, band_r2,
+                 inv_npix, order, tw);
     return EMSIM_HIP_OK;
 }
 
@@ -1430,10 +1448,11 @@ int launch_col_lensprop(float2* data, int ld, int ncols, int64_t img_stride, int
     return launch_col_mul_impl<N>(data, ld, ncols, img_stride, n_img, mul, tw, s, cd, true);
 }
 
-int spec_sparse(int n, const uint32_t* row_start, const uint16_t* bucket, const float* sf, float2* Y, int slice, int n_slices, int n_elems, int H, int ldh,
+int spec_sparse(int n, const uint32_t* row_start, const uint16_t* bucket, const float* sf,
+                const float* sft, float2* Y, int slice, int n_slices, int n_elems, int H, int ldh,
                 int n2, int bandlimit, int band_r2, float inv_npix, int batch, const int* order,
                 const float2* tw, hipStream_t s) {
-    EMSIM_DISPATCH(n, launch_spec_sparse, row_start, bucket, sf, Y, slice, n_slices,
+    EMSIM_DISPATCH(n, launch_spec_sparse, row_start, bucket, sf, sft, Y, slice, n_slices,
                    n_elems, H, ldh, n2, bandlimit, band_r2, inv_npix, batch, order, tw, s);
 }
 int slice_pass_a(int n, bool first, const float2* Y, float2* wave, int64_t n_pairs, int ldh,
@@ -1818,7 +1837,8 @@ int simulate_core(const emsim_hip_slice_builder* b, const emsim_hip_wave_propaga
     for (int sl = 0; sl < b->n_slices; ++sl) {
         const bool last = sl == b->n_slices - 1;
         if (!water.on) {
-            rc = spec_sparse(b->n1, ws.row_start, ws.bucket, b->sf, ws.Y, sl, b->n_slices,
+            rc = spec_sparse(b->n1, ws.row_start, ws.bucket, b->sf, b->sf_tiles, ws.Y, sl,
+                             b->n_slices,
                              b->n_elems, b->H, ldh, b->n2, symmetric_bandlimit ? 1 : 0, r * r,
                              inv_npix, batch, order, b->tw, s);
             if (rc) return rc;

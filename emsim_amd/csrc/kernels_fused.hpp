@@ -483,7 +483,7 @@ __device__ __forceinline__ void spec_fft(float2 (&v)[EPT], float* re, float* im,
 template <int N>
 __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_SPEC_MIN_WAVES : 1) void k_spec_sparse(
     const uint32_t* __restrict__ row_start, const uint16_t* __restrict__ bucket,
-    const float* __restrict__ sf, float2* __restrict__ Y,
+    const float* __restrict__ sf, const float* __restrict__ sft, float2* __restrict__ Y,
     int slice, int n_slices, int n_elems, int H, int ldh, int n2, int batch, int bandlimit,
     int band_r2, float inv_npix, const int* __restrict__ order, const float2* __restrict__ tw) {
     using Cfg = SpecCfg<N>;
@@ -493,8 +493,11 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
     __shared__ uint32_t lds_rows[SpecLds<N>::kRows];
     __shared__ uint16_t lds_bucket[SpecLds<N>::kBucket];
     __shared__ float2 lds_twq[SpecLds<N>::kTwq];   // w^p of the row transforms (length n2)
-    __shared__ uint32_t lds_atoms_raw[kSpecDirect > 0 ? kSpecDirect : 1];   // (x << 16) | y
-    __shared__ uint32_t lds_atoms[kSpecDirect > 0 ? kSpecDirect : 1];       // the same, sorted
+    // (x << 16) | y of a sparse plane's atoms, in bucket order -- sorted by
+    // (x, y), since rows are in x order and each row's y bins are sorted;
+    // two buffers used by alternate sparse planes, so a plane's fill never
+    // overwrites atoms another wave is still reading
+    __shared__ uint32_t lds_atoms[2][kSpecDirect > 0 ? kSpecDirect : 1];
     // the column transforms' twiddles (N <= 512; longer columns read the table,
     // as the LDS would then no longer hold two blocks per CU)
     constexpr bool kTwInLds = N <= 512;
@@ -519,6 +522,7 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
     float* im = lds_im + c * Cfg::STRIDE;
     const int tw_step = kTwN / n2;
     const int64_t sf_plane = (int64_t)N * H;
+    const int sf_tiles = (H + Cfg::C - 1) / Cfg::C;   // tiles of the tile-major table
     // the packed lane parks each element's transform Z_e in padding column
     // H + e of its molecule's Y rows (ldh >= H + n_elems)
     float2* zpark = Y + (int64_t)b * N * ldh + H;
@@ -555,6 +559,7 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
     for (int m = 0; m < EPT; ++m) acc[m] = make_float2(0.f, 0.f);
 
     bool any = false;   // block-uniform: some element has atoms in this slice
+    unsigned n_direct = 0;   // sparse planes so far (block-uniform)
     for (int e = 0; e < n_elems; ++e) {
         uint32_t e_beg, e_end;
         if (rows_lds) {
@@ -575,42 +580,35 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
             continue;
         }
         any = true;
-        // form factors first: their loads stay in flight behind the transform
-        const float* f = sf + (int64_t)e * sf_plane + col;
+        // form factors first: their loads stay in flight behind the transform.
+        // The tile-major table holds this tile's C columns side by side in
+        // every row (zeros beyond H), so a wave's load is one contiguous run.
+        const float* f = sft + (((int64_t)e * sf_tiles + tile) * N + t) * Cfg::C + c;
         float fe[EPT];
 #pragma unroll
-        for (int m = 0; m < EPT; ++m) fe[m] = valid ? f[(int64_t)(t + m * Cfg::T) * H] : 0.f;
+        for (int m = 0; m < EPT; ++m) fe[m] = f[(int64_t)m * Cfg::T * Cfg::C];
         float2 v[EPT];
         if (all_lds && n_e <= (uint32_t)kSpecDirect) {
             // few atoms: their 2-D DFT sum_a w_N^(k x_a) w_n2^(col y_a) directly,
             // cheaper than a column transform (the y factor is the twiddle the
             // row DFT would use, the x factor the transform's table entry)
+            uint32_t* atoms = lds_atoms[n_direct & 1];
+            ++n_direct;
             if (c == 0) {
 #pragma unroll
                 for (int m = 0; m < EPT; ++m) {
                     const int x = t + m * Cfg::T;
                     const uint32_t beg = lds_rows[e * N + x], end = lds_rows[e * N + x + 1];
                     for (uint32_t k = beg; k < end; ++k)
-                        lds_atoms_raw[k - e_beg] = ((uint32_t)x << 16) | lds_bucket[k - kbase];
+                        atoms[k - e_beg] = ((uint32_t)x << 16) | lds_bucket[k - kbase];
                 }
-            }
-            __syncthreads();
-            // sorted by (x, y), a fixed order for the float sum below (atoms
-            // with equal keys contribute identical terms)
-            if (threadIdx.x < n_e) {
-                const uint32_t key = lds_atoms_raw[threadIdx.x];
-                uint32_t rank = 0;
-                for (uint32_t j = 0; j < n_e; ++j) {
-                    const uint32_t o = lds_atoms_raw[j];
-                    rank += (o < key || (o == key && j < threadIdx.x)) ? 1u : 0u;
-                }
-                lds_atoms[rank] = key;
             }
             __syncthreads();
 #pragma unroll
             for (int m = 0; m < EPT; ++m) v[m] = make_float2(0.f, 0.f);
+            // in (x, y) order: a fixed order for the float sum
             for (uint32_t a = 0; a < n_e; ++a) {
-                const uint32_t xy = lds_atoms[a];
+                const uint32_t xy = atoms[a];
                 const int x = (int)(xy >> 16), y = (int)(xy & 0xffffu);
                 float2 wy = lds_twq[(col * y) & (n2 - 1)];
                 if (packed)   // column 0 (1) + i column n2/2 ((-1)^y), both real
@@ -625,7 +623,6 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
                     v[m].y += wx.x * wy.y + wx.y * wy.x;
                 }
             }
-            __syncthreads();   // the next element rebuilds lds_atoms
         } else {
             // block-uniform choice, so each path's loads have one address space
             if (all_lds)

@@ -383,13 +383,24 @@ __global__ __launch_bounds__(RowCfg<N>::BLOCK) void k_row_c2r_pairs(OpC2RPairs<N
 // complex apart (H padded to a multiple of 16, so every tile's segments are
 // 64-byte aligned); the form-factor table rows are H apart.
 // ---------------------------------------------------------------------------
+// Columns per tile of the spectrum column kernels for length-n columns
+// (SpecCfg<n>::C); the host lays out their tile-major form-factor table by it.
+__host__ __device__ constexpr int spec_tile_cols(int n) {
+    return (512 / (n / ((n / 8) > 256 ? n / 256 : 8))) < 1
+               ? 1
+               : ((512 / (n / ((n / 8) > 256 ? n / 256 : 8))) < 8
+                      ? (512 / (n / ((n / 8) > 256 ? n / 256 : 8)))
+                      : 8);
+}
+
 template <int N>
 struct SpecCfg {
     // 8 values per thread and 8 columns per tile: 512-thread blocks, room
     // for the accumulator next to the transform without spilling
     static constexpr int EPT = (N / 8) > 256 ? N / 256 : 8;
     static constexpr int T = N / EPT;
-    static constexpr int C = (512 / T) < 1 ? 1 : ((512 / T) < 8 ? (512 / T) : 8);
+    static constexpr int C = spec_tile_cols(N);
+    static_assert(C == ((512 / T) < 1 ? 1 : ((512 / T) < 8 ? (512 / T) : 8)), "tile width");
     static constexpr int BLOCK = T * C;
     static constexpr int STRIDE = LdsLayout<N>::kStride;
 };
