@@ -528,30 +528,40 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
     float2* zpark = Y + (int64_t)b * N * ldh + H;
 
     // stage this (molecule, slice)'s rows (rows of all elements are adjacent)
+    // and its atoms' y bins.  The bucket range comes straight from global
+    // memory (uniform addresses, so scalar loads; readfirstlane keeps the
+    // values in scalar registers and the branches on them scalar), so every
+    // staging load of a thread -- row table, bucket, twiddles -- is in flight
+    // before its first LDS store, and one barrier ends the staging.
     const int64_t rbase = (((int64_t)b * n_slices + slice) * n_elems) * N;
     const int n_rows = n_elems * N;
     const bool rows_lds = n_rows < SpecLds<N>::kRows;
-    if (rows_lds)
-        stage_lds<4>(lds_rows, row_start + rbase, (uint32_t)n_rows + 1, threadIdx.x, Cfg::BLOCK);
     const bool tw_lds = n2 <= SpecLds<N>::kTwq;
-    if (tw_lds)
-        for (int i = threadIdx.x; i < n2; i += Cfg::BLOCK) lds_twq[i] = tw[i * tw_step];
-    if constexpr (kTwInLds) TwLds<N>::fill(lds_tw, tw, threadIdx.x, Cfg::BLOCK);
-    __syncthreads();
-    // block-uniform values read per lane: readfirstlane puts them in scalar
-    // registers, so the branches on them are scalar branches
-    uint32_t kbase, kend;
-    if (rows_lds) {
-        kbase = __builtin_amdgcn_readfirstlane(lds_rows[0]);
-        kend = __builtin_amdgcn_readfirstlane(lds_rows[n_rows]);
-    } else {
-        kbase = __builtin_amdgcn_readfirstlane(row_start[rbase]);
-        kend = __builtin_amdgcn_readfirstlane(row_start[rbase + n_rows]);
-    }
+    const uint32_t kbase = __builtin_amdgcn_readfirstlane(row_start[rbase]);
+    const uint32_t kend = __builtin_amdgcn_readfirstlane(row_start[rbase + n_rows]);
     const uint32_t kcount = kend - kbase;
     // everything in LDS (the usual case), or the bucket in pieces
     const bool all_lds = rows_lds && tw_lds && kcount <= (uint32_t)SpecLds<N>::kBucket;
+    constexpr int RP = (SpecLds<N>::kRows + Cfg::BLOCK - 1) / Cfg::BLOCK;
+    uint32_t rv[RP];
+    if (rows_lds) {
+#pragma unroll
+        for (int j = 0; j < RP; ++j) {
+            const int i = (int)threadIdx.x + j * Cfg::BLOCK;
+            rv[j] = row_start[rbase + (i <= n_rows ? i : n_rows)];
+        }
+    }
     if (all_lds) stage_lds<4>(lds_bucket, bucket + kbase, kcount, threadIdx.x, Cfg::BLOCK);
+    if (rows_lds) {
+#pragma unroll
+        for (int j = 0; j < RP; ++j) {
+            const int i = (int)threadIdx.x + j * Cfg::BLOCK;
+            if (i <= n_rows) lds_rows[i] = rv[j];
+        }
+    }
+    if (tw_lds)
+        for (int i = threadIdx.x; i < n2; i += Cfg::BLOCK) lds_twq[i] = tw[i * tw_step];
+    if constexpr (kTwInLds) TwLds<N>::fill(lds_tw, tw, threadIdx.x, Cfg::BLOCK);
     __syncthreads();
 
     float2 acc[EPT];
