@@ -439,8 +439,17 @@ int launch_rows_mode(const FnoRowArgs& a, int mode, dim3 grid, size_t lds, hipSt
     return EMSIM_HIP_OK;
 }
 
-int launch_rows(const FnoRowArgs& a, int mode, int64_t rows, hipStream_t s) {
-    const dim3 grid((unsigned)rows);
+// Rows per row-kernel block: the Gy / Fy fragments and W a block loads serve
+// this many rows.
+#ifndef EMSIM_FNO_RPB
+#define EMSIM_FNO_RPB 16
+#endif
+
+int launch_rows(const FnoRowArgs& a_in, int mode, int64_t rows, hipStream_t s) {
+    FnoRowArgs a = a_in;
+    a.rows = rows;
+    a.rpb = EMSIM_FNO_RPB;
+    const dim3 grid((unsigned)((rows + a.rpb - 1) / a.rpb));
     const size_t lds = kFnoRowLds(a.n);
     if (mode == kModeLiftPacked) {
         if (a.k1 == 16 && a.m2x2 == 16)

@@ -202,6 +202,8 @@ struct FnoRowArgs {
     const float* slices;   // (B, ns, n, n)
     int ns;
     float inv_psi0, sigma;
+    int64_t rows;          // rows of the batch (B n)
+    int rpb;               // rows per block
 };
 
 constexpr int kFnoWStride = 72;    // bf16 per LDS row of W^T / Uhat^T (144 B, 16-B aligned)
@@ -289,13 +291,14 @@ __global__ __launch_bounds__(256) void k_fno_rows(FnoRowArgs a) {
     uint16_t* vT = uT + 32 * kFnoWStride;           // [c][y]
     constexpr int k1 = 16 * K1F, m2x2 = 16 * K2F;
     const int n = a.n, vts = fno_vt_stride(n);
-    const int64_t row = blockIdx.x;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int l32 = lane & 31, h = lane >> 5;
-    const uint16_t* vrow = a.vin + row * n * k1;
+    // this block's rows; the Gy and Fy fragments and W are the same for
+    // every row, so they are loaded and staged once per block
+    const int64_t r_beg = (int64_t)blockIdx.x * a.rpb;
+    const int64_t r_end = r_beg + a.rpb < a.rows ? r_beg + a.rpb : a.rows;
 
-    bf16x8_t vf[4][K1F], gf[4][K2F], ff[4][2];
-    if constexpr (MODE & kRowPack) fno_packed_frags<K1F>(a, row, w, l32, h, vf);
+    bf16x8_t gf[4][K2F], ff[4][2];
     // Without the next y-transform (TOUT) the loads are unconditional, from
     // clamped indices and masked afterwards, so all of a wave's fragment
     // loads are in flight together instead of each tile's waiting behind its
@@ -307,18 +310,6 @@ __global__ __launch_bounds__(256) void k_fno_rows(FnoRowArgs a) {
         const int y0 = 32 * (w + 4 * i);
         const bool ok = y0 < n;
         const int y = (UNC && !ok ? 0 : y0) + l32;
-        if constexpr (!(MODE & kRowPack)) {
-#pragma unroll
-            for (int f = 0; f < K1F; ++f) {
-                if constexpr (UNC) {
-                    const bf16x8_t t = *(const bf16x8_t*)(vrow + (int64_t)y * k1 + 16 * f + 8 * h);
-                    vf[i][f] = ok ? t : bf16x8_t{};
-                } else {
-                    vf[i][f] = bf16x8_t{};
-                    if (ok) vf[i][f] = *(const bf16x8_t*)(vrow + (int64_t)y * k1 + 16 * f + 8 * h);
-                }
-            }
-        }
 #pragma unroll
         for (int f = 0; f < K2F; ++f) {
             gf[i][f] = bf16x8_t{};
@@ -342,6 +333,33 @@ __global__ __launch_bounds__(256) void k_fno_rows(FnoRowArgs a) {
         const int k = i >> 5, c = i & 31;
         wT[c * kFnoWStride + k] = a.w16[i];
     }
+    if (threadIdx.x < 32) lds_bias[threadIdx.x] = a.bias[threadIdx.x];
+    if (PROJ && threadIdx.x < 64) lds_q[threadIdx.x] = bf16_to_f32(a.q16[threadIdx.x]);
+
+  for (int64_t row = r_beg; row < r_end; ++row) {
+    const uint16_t* vrow = a.vin + row * n * k1;
+    bf16x8_t vf[4][K1F];
+    if constexpr (MODE & kRowPack) fno_packed_frags<K1F>(a, row, w, l32, h, vf);
+    if constexpr (!(MODE & kRowPack)) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int y0 = 32 * (w + 4 * i);
+            const bool ok = y0 < n;
+            const int y = (UNC && !ok ? 0 : y0) + l32;
+#pragma unroll
+            for (int f = 0; f < K1F; ++f) {
+                if constexpr (UNC) {
+                    const bf16x8_t t = *(const bf16x8_t*)(vrow + (int64_t)y * k1 + 16 * f + 8 * h);
+                    vf[i][f] = ok ? t : bf16x8_t{};
+                } else {
+                    vf[i][f] = bf16x8_t{};
+                    if (ok) vf[i][f] = *(const bf16x8_t*)(vrow + (int64_t)y * k1 + 16 * f + 8 * h);
+                }
+            }
+        }
+    }
+    // the previous row is done with uT and with vT (its partial sums of T)
+    __syncthreads();
     if (HAS_U) {
         const uint16_t* u = a.U + row * m2x2 * 32;
         for (int i = threadIdx.x; i < m2x2 * 32; i += 256) {
@@ -349,8 +367,6 @@ __global__ __launch_bounds__(256) void k_fno_rows(FnoRowArgs a) {
             uT[c * kFnoWStride + j] = u[i];
         }
     }
-    if (threadIdx.x < 32) lds_bias[threadIdx.x] = a.bias[threadIdx.x];
-    if (PROJ && threadIdx.x < 64) lds_q[threadIdx.x] = bf16_to_f32(a.q16[threadIdx.x]);
     __syncthreads();
 
 #pragma unroll
@@ -404,7 +420,7 @@ __global__ __launch_bounds__(256) void k_fno_rows(FnoRowArgs a) {
             if (h == 0) *(float2*)(a.out + pix * 2) = make_float2(p0 + a.qb[0], p1 + a.qb[1]);
         }
     }
-    if (!TOUT) return;
+    if (!TOUT) continue;
     __syncthreads();
     // T^T (c, m2) = sum_y v'^T (c, y) Fy^T (y, m2), each wave over its tiles
     f32x16_t acc = {};
@@ -438,6 +454,7 @@ __global__ __launch_bounds__(256) void k_fno_rows(FnoRowArgs a) {
         }
         trow[i] = f32x2_to_bf16x2(s0, s1);
     }
+  }
 }
 
 // ---------------------------------------------------------------------------
