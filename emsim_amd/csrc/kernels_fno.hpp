@@ -281,8 +281,17 @@ __device__ __forceinline__ void fno_packed_frags(const FnoRowArgs& a, int64_t ro
     }
 }
 
+// Minimum waves per SIMD the row kernel is compiled for: three (168
+// VGPRs) where that costs no spills -- the lifting from 16 channels, the last
+// layer, the first layer -- measured faster there; the middle layers (the
+// Uhat term and the next y-transform) keep the compiler's choice, as do
+// 32-channel packed liftings.
+__host__ __device__ constexpr int fno_rows_min_waves(int k1f, int mode) {
+    return (((mode & kRowU) && (mode & kRowT)) || ((mode & kRowPack) && k1f > 1)) ? 1 : 3;
+}
+
 template <int K1F, int K2F, int MODE>
-__global__ __launch_bounds__(256) void k_fno_rows(FnoRowArgs a) {
+__global__ __launch_bounds__(256, fno_rows_min_waves(K1F, MODE)) void k_fno_rows(FnoRowArgs a) {
     constexpr bool HAS_U = MODE & kRowU, ACT = MODE & kRowAct, VOUT = MODE & kRowV,
                    ZOUT = MODE & kRowZ, TOUT = MODE & kRowT, PROJ = MODE & kRowProj;
     extern __shared__ __align__(16) uint16_t fno_lds[];
