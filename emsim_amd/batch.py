@@ -78,15 +78,88 @@ class RawBatch:
         return len(self.labels)
 
 
-def centred_thickness(mol: atm.AtomList) -> float:
-    """z extent of ``atoms.centralize(mol)`` (the slab thickness the
-    reference sizes its slices from), without centring the other axes."""
-    z = np.asarray(mol.coordinates[:, 0], dtype=np.float32)
+def _centred_extent(z: np.ndarray) -> float:
+    """Extent of float32 coordinates z after the centring shift
+    ``atoms.centralize`` applies (computed in float32 as it does)."""
     if z.size == 0:
         return 0.0
     lo, hi = z.min(), z.max()
     zc = z + (-lo - (hi - lo) / np.float32(2.0))
     return float(zc.max() - zc.min())
+
+
+def centred_thickness(mol: atm.AtomList) -> float:
+    """z extent of ``atoms.centralize(mol)`` (the slab thickness the
+    reference sizes its slices from), without centring the other axes."""
+    return _centred_extent(np.asarray(mol.coordinates[:, 0], dtype=np.float32))
+
+
+def element_indices(mol: atm.AtomList, elements: Sequence[int]) -> np.ndarray:
+    """(n_atoms,) uint8 index of each atom's element in ``elements``;
+    ValueError for an element outside it."""
+    z = np.asarray(mol.elements, dtype=np.int64)
+    lut = np.full(max(int(z.max(initial=0)), max(elements)) + 1, 255, dtype=np.uint8)
+    lut[list(elements)] = np.arange(len(elements), dtype=np.uint8)
+    if z.size and (z.min() < 0 or (lut[z] == 255).any()):
+        unknown = sorted(set(np.unique(z).tolist()) - set(elements))
+        raise ValueError(f"molecule has elements {unknown} outside the pipe's element "
+                         f"list {list(elements)}")
+    return lut[z]
+
+
+class RawStaging:
+    """Pinned host buffers a raw batch is packed into, and the event of the
+    copies last queued from them (waited for before they are refilled)."""
+
+    def __init__(self):
+        self.coords = self.elem = self.done = None
+
+    def acquire(self, n_atoms: int) -> "RawStaging":
+        if self.done is not None:
+            self.done.synchronize()
+        if self.coords is None or self.coords.shape[0] < n_atoms:
+            cap = max(int(n_atoms), 1)
+            self.coords = torch.empty((cap, 3), dtype=torch.float32, pin_memory=True)
+            self.elem = torch.empty((cap,), dtype=torch.uint8, pin_memory=True)
+            self.done = torch.cuda.Event()
+        return self
+
+    def fill(self, mols: Sequence[atm.AtomList], elements: Sequence[int],
+             thickness: bool = False):
+        """Write the molecules' coordinates (as given) and element indices
+        into the buffers, one slice per molecule on a thread pool; returns
+        the molecule offsets and, with ``thickness``, each molecule's
+        centred z extent (``centred_thickness``) read off the copy."""
+        if len(mols) == 0:
+            raise ValueError("empty batch")
+        if len(elements) > 254:
+            raise ValueError("at most 254 elements per pipe")
+        sizes = np.array([len(m) for m in mols], dtype=np.int64)
+        mol_off = np.zeros(len(mols) + 1, dtype=np.int64)
+        np.cumsum(sizes, out=mol_off[1:])
+        self.acquire(int(mol_off[-1]))
+        ch, eh = self.coords.numpy(), self.elem.numpy()
+        thick = [0.0] * len(mols)
+
+        def one(i):
+            a, b = int(mol_off[i]), int(mol_off[i + 1])
+            eh[a:b] = element_indices(mols[i], elements)
+            c = ch[a:b]
+            c[...] = np.asarray(mols[i].coordinates, dtype=np.float32).reshape(-1, 3)
+            if thickness:
+                thick[i] = _centred_extent(c[:, 0])
+
+        pool_map(one, range(len(mols)))
+        return mol_off, thick
+
+    def upload(self, mol_off: np.ndarray, labels) -> RawBatch:
+        """Queue the copies of the filled buffers on the current stream
+        (without waiting for them) and record their event."""
+        n, dev = int(mol_off[-1]), device()
+        coords = self.coords[:n].to(dev, non_blocking=True)
+        elem = self.elem[:n].to(dev, non_blocking=True)
+        self.done.record()
+        return RawBatch(coords, elem, mol_off, list(labels))
 
 
 class MixedBatch:
@@ -105,6 +178,7 @@ class MixedBatch:
         self.elements = sorted(int(z) for z in elements)
         self.symmetric_bandlimit = symmetric_bandlimit
         self._pipes = {}
+        self._stages = {}   # wave size -> RawStaging
         self.last_groups = []
 
     def pipe(self, n: int, n_slices: int) -> "BatchPipe":
@@ -136,19 +210,33 @@ class MixedBatch:
         out: List[torch.Tensor] = [None] * len(items)
         self.last_groups = []
         mols = [m for m, _ in items]
+        groups = [(n, [i for i, (_, sz) in enumerate(items) if int(sz) == n])
+                  for n in sorted({int(sz) for _, sz in items})]
         if device_prep:
-            thick = pool_map(centred_thickness, mols)
-        else:   # host preparation of every molecule on a thread pool
-            mols = pool_map(atm.centralize, mols)
-            thick = pool_map(lambda m: float(m.space[0]), mols)
-        for n in sorted({int(sz) for _, sz in items}):
-            idx = [i for i, (_, sz) in enumerate(items) if int(sz) == n]
+            # each group's host work -- one pass that packs its atoms into
+            # pinned staging and reads off every molecule's thickness -- is
+            # done just before its launches, so it overlaps the GPU's work on
+            # the group before; the group with the fewest atoms goes first
+            groups.sort(key=lambda g: sum(len(mols[i]) for i in g[1]))
+            for n, idx in groups:
+                st = self._stages.setdefault(n, RawStaging())
+                group = [mols[i] for i in idx]
+                mol_off, thick = st.fill(group, self.elements, thickness=True)
+                ns = self._slices_for(max(thick), n)
+                p = self.pipe(n, ns)
+                res = p._simulate_raw(st.upload(mol_off, [getattr(m, "label", "")
+                                                          for m in group]), mode)
+                for j, i in enumerate(idx):
+                    out[i] = res[j]
+                self.last_groups.append((n, ns, idx))
+            return out
+        # host preparation of every molecule on a thread pool
+        mols = pool_map(atm.centralize, mols)
+        thick = pool_map(lambda m: float(m.space[0]), mols)
+        for n, idx in groups:
             ns = self._slices_for(max(thick[i] for i in idx), n)
             p = self.pipe(n, ns)
-            if device_prep:
-                res = p._simulate_raw(p.upload_raw([mols[i] for i in idx]), mode)
-            else:
-                res = p._simulate(p.upload(p.prepare_many([mols[i] for i in idx])), mode)
+            res = p._simulate(p.upload(p.prepare_many([mols[i] for i in idx])), mode)
             for j, i in enumerate(idx):
                 out[i] = res[j]
             self.last_groups.append((n, ns, idx))
@@ -228,14 +316,7 @@ class BatchPipe:
 
     def element_indices(self, mol: atm.AtomList) -> np.ndarray:
         """(n_atoms,) uint8 index of each atom's element in the pipe's list."""
-        z = np.asarray(mol.elements, dtype=np.int64)
-        lut = np.full(max(int(z.max(initial=0)), max(self.elements)) + 1, 255, dtype=np.uint8)
-        lut[self.elements] = np.arange(len(self.elements), dtype=np.uint8)
-        if z.size and (z.min() < 0 or (lut[z] == 255).any()):
-            unknown = sorted(set(np.unique(z).tolist()) - set(self.elements))
-            raise ValueError(f"molecule has elements {unknown} outside the pipe's element "
-                             f"list {self.elements}")
-        return lut[z]
+        return element_indices(mol, self.elements)
 
     def pack_raw(self, mols: Sequence[atm.AtomList], elem=None):
         """A raw batch as plain host arrays -- (coordinates (n, 3) float32 as
@@ -262,14 +343,10 @@ class BatchPipe:
         staging buffers, without waiting for the copy."""
         coords, elem, mol_off, labels = packed
         n = int(mol_off[-1])
-        st = self._staging(n)
-        np.copyto(st["coords"].numpy()[:n], np.asarray(coords, dtype=np.float32).reshape(-1, 3))
-        np.copyto(st["elem"].numpy()[:n], np.asarray(elem, dtype=np.uint8))
-        dev = device()
-        c = st["coords"][:n].to(dev, non_blocking=True)
-        e = st["elem"][:n].to(dev, non_blocking=True)
-        st["done"].record()
-        return RawBatch(c, e, mol_off, list(labels))
+        st = self._raw_staging().acquire(n)
+        np.copyto(st.coords.numpy()[:n], np.asarray(coords, dtype=np.float32).reshape(-1, 3))
+        np.copyto(st.elem.numpy()[:n], np.asarray(elem, dtype=np.uint8))
+        return st.upload(mol_off, labels)
 
     def upload_raw(self, mols: Sequence[atm.AtomList]) -> RawBatch:
         """Molecules as they come, copied to the GPU once: coordinates as
@@ -282,41 +359,14 @@ class BatchPipe:
         the next batch overlaps this one's simulation.  The buffers belong
         to this pipe; the next call waits for these copies before refilling
         them."""
-        if len(mols) == 0:
-            raise ValueError("empty batch")
-        if len(self.elements) > 254:
-            raise ValueError("at most 254 elements per pipe")
-        sizes = np.array([len(m) for m in mols], dtype=np.int64)
-        mol_off = np.zeros(len(mols) + 1, dtype=np.int64)
-        np.cumsum(sizes, out=mol_off[1:])
-        n = int(mol_off[-1])
-        st = self._staging(n)
-        ch, eh = st["coords"].numpy(), st["elem"].numpy()
+        st = self._raw_staging()
+        mol_off, _ = st.fill(mols, self.elements)
+        return st.upload(mol_off, [getattr(m, "label", "") for m in mols])
 
-        def fill(i):
-            a, b = int(mol_off[i]), int(mol_off[i + 1])
-            eh[a:b] = self.element_indices(mols[i])
-            ch[a:b] = np.asarray(mols[i].coordinates, dtype=np.float32).reshape(-1, 3)
-
-        pool_map(fill, range(len(mols)))
-        dev = device()
-        coords = st["coords"][:n].to(dev, non_blocking=True)
-        elem = st["elem"][:n].to(dev, non_blocking=True)
-        st["done"].record()
-        return RawBatch(coords, elem, mol_off, [getattr(m, "label", "") for m in mols])
-
-    def _staging(self, n_atoms: int):
-        """Pinned host buffers of upload_raw for n_atoms atoms, once the
-        copies queued from them by the previous call are done."""
+    def _raw_staging(self) -> RawStaging:
         st = getattr(self, "_stage", None)
-        if st is not None:
-            st["done"].synchronize()
-        if st is None or st["coords"].shape[0] < n_atoms:
-            cap = max(int(n_atoms), 1)
-            st = self._stage = {
-                "coords": torch.empty((cap, 3), dtype=torch.float32, pin_memory=True),
-                "elem": torch.empty((cap,), dtype=torch.uint8, pin_memory=True),
-                "done": torch.cuda.Event()}
+        if st is None:
+            st = self._stage = RawStaging()
         return st
 
     def upload_orientations(self, mol: atm.AtomList, quats: np.ndarray) -> PreparedBatch:
