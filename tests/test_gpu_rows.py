@@ -277,3 +277,31 @@ def test_device_preparation_equals_host_preparation():
     bad = atm.AtomList(np.array([6, 79]), np.zeros((2, 3), np.float32))
     with pytest.raises(ValueError, match="outside"):
         bp.upload_raw([mols[0], bad])
+
+
+def test_raw_staging_reused_across_calls():
+    """Raw uploads go through pinned staging buffers that the next upload
+    refills: two batches uploaded back to back before either runs, a larger
+    batch after a smaller one (the buffers grow), and MixedBatch run twice in
+    a row (its per-size buffers reused) all give the images of separate,
+    fully synchronised runs."""
+    tem = _tem()
+    quats = random_uniform_quaternions(5, rng=np.random.default_rng(341))
+    mols = [synth.rotated(synth.protein_cloud(n, 340 + i), quats[i])
+            for i, n in enumerate((4000, 7000, 2500, 12000, 5000))]
+    bp = BatchPipe(tem, 1.0, 2.0, 128, 16, ELEMS)
+    ref = [bp.run(bp.upload([bp.prepare(m)]))[0].clone() for m in mols]
+    torch.cuda.synchronize()
+    r1 = bp.upload_raw(mols[:2])
+    r2 = bp.upload_raw(mols[2:4])          # refills the staging r1 came from
+    big = bp.upload_raw(mols)              # grows the buffers
+    out1, out2, out3 = bp.run_raw(r1), bp.run_raw(r2), bp.run_raw(big)
+    for j in range(2):
+        assert torch.equal(out1[j], ref[j]) and torch.equal(out2[j], ref[2 + j])
+    for j in range(5):
+        assert torch.equal(out3[j], ref[j])
+    items = [(m, n) for m, n in zip(mols, (128, 256, 128, 256, 128))]
+    mb = MixedBatch(tem, 1.0, 2.0, ELEMS)
+    first = [x.clone() for x in mb.run(items)]
+    second = mb.run(items)
+    assert all(torch.equal(x, y) for x, y in zip(first, second))
