@@ -281,13 +281,16 @@ __device__ __forceinline__ void fno_packed_frags(const FnoRowArgs& a, int64_t ro
     }
 }
 
-// Minimum waves per SIMD the row kernel is compiled for: three (168
-// VGPRs) where that costs no spills -- the lifting from 16 channels, the last
-// layer, the first layer -- measured faster there; the middle layers (the
-// Uhat term and the next y-transform) keep the compiler's choice, as do
-// 32-channel packed liftings.
+// Minimum waves per SIMD the row kernel is compiled for: two (256 VGPRs,
+// room for two rows' fragments without spills) for the lifting and the last
+// layer; the middle layers (the Uhat term and the next y-transform), the
+// 32-channel packed liftings and the training modes (z written) keep the
+// compiler's choice.
 __host__ __device__ constexpr int fno_rows_min_waves(int k1f, int mode) {
-    return (((mode & kRowU) && (mode & kRowT)) || ((mode & kRowPack) && k1f > 1)) ? 1 : 3;
+    return (((mode & kRowU) && (mode & kRowT)) || ((mode & kRowPack) && k1f > 1) ||
+            (mode & kRowZ))
+               ? 1
+               : 2;
 }
 
 template <int K1F, int K2F, int MODE>
@@ -338,132 +341,164 @@ __global__ __launch_bounds__(256, fno_rows_min_waves(K1F, MODE)) void k_fno_rows
     }
 
     __shared__ float lds_bias[32], lds_q[64];
-    for (int i = threadIdx.x; i < k1 * 32; i += 256) {
-        const int k = i >> 5, c = i & 31;
-        wT[c * kFnoWStride + k] = a.w16[i];
+    // W (k1 x 32 bf16) as four channels per thread, all loads at once
+    static_assert(k1 * 8 <= 256, "one group of W per thread");
+    if ((int)threadIdx.x < k1 * 8) {
+        const uint2 wq = *(const uint2*)(a.w16 + 4 * threadIdx.x);
+        const int k = (4 * (int)threadIdx.x) >> 5, c = (4 * (int)threadIdx.x) & 31;
+        wT[c * kFnoWStride + k] = (uint16_t)wq.x;
+        wT[(c + 1) * kFnoWStride + k] = (uint16_t)(wq.x >> 16);
+        wT[(c + 2) * kFnoWStride + k] = (uint16_t)wq.y;
+        wT[(c + 3) * kFnoWStride + k] = (uint16_t)(wq.y >> 16);
     }
     if (threadIdx.x < 32) lds_bias[threadIdx.x] = a.bias[threadIdx.x];
     if (PROJ && threadIdx.x < 64) lds_q[threadIdx.x] = bf16_to_f32(a.q16[threadIdx.x]);
 
-  for (int64_t row = r_beg; row < r_end; ++row) {
-    const uint16_t* vrow = a.vin + row * n * k1;
-    bf16x8_t vf[4][K1F];
-    if constexpr (MODE & kRowPack) fno_packed_frags<K1F>(a, row, w, l32, h, vf);
-    if constexpr (!(MODE & kRowPack)) {
+    // The rows are software-pipelined in pairs: the next row's input
+    // fragments and Uhat row are loaded while the current row computes.
+    // Every global load is unconditional (indices clamped into range), so
+    // the wait before a row's first use of its fragments can leave the next
+    // row's loads in flight; a load behind a branch would force a wait for
+    // everything outstanding.
+    constexpr int kU4 = m2x2 * 8;   // 4-channel groups of one Uhat row (2 M2 x 32 bf16)
+    static_assert(kU4 <= 256, "one group of a Uhat row per thread");
+    const bool u_lane = HAS_U && (int)threadIdx.x < kU4;
+    const int u_grp = (int)threadIdx.x % kU4;
+    // input fragments of one row; a tile beyond n reads the row's first
+    // tile and is never used
+    auto load_v = [&](int64_t row, bf16x8_t (&vf)[4][K1F]) __attribute__((always_inline)) {
+        if constexpr (MODE & kRowPack) {
+            fno_packed_frags<K1F>(a, row, w, l32, h, vf);
+        } else {
+            const uint16_t* vrow = a.vin + row * n * k1;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int y0 = 32 * (w + 4 * i);
+                const bool ok = y0 < n;
+                const int y = (ok ? y0 : 0) + l32;
+#pragma unroll
+                for (int f = 0; f < K1F; ++f)
+                    vf[i][f] = *(const bf16x8_t*)(vrow + (int64_t)y * k1 + 16 * f + 8 * h);
+            }
+        }
+    };
+    auto load_u = [&](int64_t row) __attribute__((always_inline)) {
+        return HAS_U ? *(const uint2*)(a.U + row * m2x2 * 32 + 4 * u_grp) : make_uint2(0u, 0u);
+    };
+    auto body = [&](int64_t row, const bf16x8_t (&vf)[4][K1F], uint2 u_cur) __attribute__((always_inline)) {
+        // the previous row is done with uT and with vT (its partial sums of T)
+        __syncthreads();
+        if (u_lane) {
+            // channels c .. c + 3 of mode j
+            const int j = (4 * (int)threadIdx.x) >> 5, c = (4 * (int)threadIdx.x) & 31;
+            uT[c * kFnoWStride + j] = (uint16_t)u_cur.x;
+            uT[(c + 1) * kFnoWStride + j] = (uint16_t)(u_cur.x >> 16);
+            uT[(c + 2) * kFnoWStride + j] = (uint16_t)u_cur.y;
+            uT[(c + 3) * kFnoWStride + j] = (uint16_t)(u_cur.y >> 16);
+        }
+        __syncthreads();
+
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int y0 = 32 * (w + 4 * i), y = y0 + l32;
+            if (y0 >= n) break;
+            f32x16_t acc = {};
+#pragma unroll
+            for (int f = 0; f < K1F; ++f) {
+                const bf16x8_t fa = *(const bf16x8_t*)(wT + l32 * kFnoWStride + 16 * f + 8 * h);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa, vf[i][f], acc, 0, 0, 0);
+            }
+            if (HAS_U) {
+#pragma unroll
+                for (int f = 0; f < K2F; ++f) {
+                    const bf16x8_t fa = *(const bf16x8_t*)(uT + l32 * kFnoWStride + 16 * f + 8 * h);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa, gf[i][f], acc, 0, 0, 0);
+                }
+            }
+            const int64_t pix = row * n + y;
+            float p0 = 0.f, p1 = 0.f;
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int c0 = 8 * g + 4 * h;
+                float z[4], v[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    z[j] = acc[4 * g + j] + lds_bias[c0 + j];
+                    v[j] = ACT ? gelu(z[j]) : z[j];
+                }
+                const uint2 vb = make_uint2(f32x2_to_bf16x2(v[0], v[1]), f32x2_to_bf16x2(v[2], v[3]));
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const uint32_t word = j < 2 ? vb.x : vb.y;
+                    const uint16_t bits = (uint16_t)((j & 1) ? word >> 16 : word);
+                    vT[(c0 + j) * vts + y] = bits;
+                    if (PROJ) {
+                        const float vv = bf16_to_f32(bits);
+                        p0 += vv * lds_q[2 * (c0 + j)];
+                        p1 += vv * lds_q[2 * (c0 + j) + 1];
+                    }
+                }
+                if (ZOUT)
+                    *(uint2*)(a.zout + pix * 32 + c0) =
+                        make_uint2(f32x2_to_bf16x2(z[0], z[1]), f32x2_to_bf16x2(z[2], z[3]));
+                if (VOUT) *(uint2*)(a.vout + pix * 32 + c0) = vb;
+            }
+            if (PROJ) {
+                p0 += __shfl_xor(p0, 32);
+                p1 += __shfl_xor(p1, 32);
+                if (h == 0) *(float2*)(a.out + pix * 2) = make_float2(p0 + a.qb[0], p1 + a.qb[1]);
+            }
+        }
+        if (!TOUT) return;
+        __syncthreads();
+        // T^T (c, m2) = sum_y v'^T (c, y) Fy^T (y, m2), each wave over its tiles
+        f32x16_t acc = {};
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int y0 = 32 * (w + 4 * i);
-            const bool ok = y0 < n;
-            const int y = (UNC && !ok ? 0 : y0) + l32;
+            if (y0 >= n) break;
 #pragma unroll
-            for (int f = 0; f < K1F; ++f) {
-                if constexpr (UNC) {
-                    const bf16x8_t t = *(const bf16x8_t*)(vrow + (int64_t)y * k1 + 16 * f + 8 * h);
-                    vf[i][f] = ok ? t : bf16x8_t{};
-                } else {
-                    vf[i][f] = bf16x8_t{};
-                    if (ok) vf[i][f] = *(const bf16x8_t*)(vrow + (int64_t)y * k1 + 16 * f + 8 * h);
-                }
+            for (int f = 0; f < 2; ++f) {
+                const bf16x8_t fa = *(const bf16x8_t*)(vT + l32 * vts + y0 + 16 * f + 8 * h);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa, ff[i][f], acc, 0, 0, 0);
             }
         }
-    }
-    // the previous row is done with uT and with vT (its partial sums of T)
-    __syncthreads();
-    if (HAS_U) {
-        const uint16_t* u = a.U + row * m2x2 * 32;
-        for (int i = threadIdx.x; i < m2x2 * 32; i += 256) {
-            const int j = i >> 5, c = i & 31;
-            uT[c * kFnoWStride + j] = u[i];
+        __syncthreads();   // vT is reused for the partial sums of the four waves
+        float* red = (float*)vT;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int c = 8 * (v >> 2) + 4 * h + (v & 3);
+            red[(w * 32 + c) * 33 + l32] = acc[v];
         }
-    }
-    __syncthreads();
-
+        __syncthreads();
+        // T leaves as bf16: its one consumer, the x-transform, rounds it to bf16
+        uint32_t* trow = (uint32_t*)(a.T + row * m2x2 * 32);
+        for (int i = threadIdx.x; i < m2x2 * 16; i += 256) {
+            const int m = i >> 4, c = 2 * (i & 15);
+            float s0 = 0.f, s1 = 0.f;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int y0 = 32 * (w + 4 * i), y = y0 + l32;
-        if (y0 >= n) break;
-        f32x16_t acc = {};
-#pragma unroll
-        for (int f = 0; f < K1F; ++f) {
-            const bf16x8_t fa = *(const bf16x8_t*)(wT + l32 * kFnoWStride + 16 * f + 8 * h);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa, vf[i][f], acc, 0, 0, 0);
-        }
-        if (HAS_U) {
-#pragma unroll
-            for (int f = 0; f < K2F; ++f) {
-                const bf16x8_t fa = *(const bf16x8_t*)(uT + l32 * kFnoWStride + 16 * f + 8 * h);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa, gf[i][f], acc, 0, 0, 0);
+            for (int q = 0; q < 4; ++q) {
+                s0 += red[(q * 32 + c) * 33 + m];
+                s1 += red[(q * 32 + c + 1) * 33 + m];
             }
+            trow[i] = f32x2_to_bf16x2(s0, s1);
         }
-        const int64_t pix = row * n + y;
-        float p0 = 0.f, p1 = 0.f;
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            const int c0 = 8 * g + 4 * h;
-            float z[4], v[4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                z[j] = acc[4 * g + j] + lds_bias[c0 + j];
-                v[j] = ACT ? gelu(z[j]) : z[j];
-            }
-            const uint2 vb = make_uint2(f32x2_to_bf16x2(v[0], v[1]), f32x2_to_bf16x2(v[2], v[3]));
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const uint32_t word = j < 2 ? vb.x : vb.y;
-                const uint16_t bits = (uint16_t)((j & 1) ? word >> 16 : word);
-                vT[(c0 + j) * vts + y] = bits;
-                if (PROJ) {
-                    const float vv = bf16_to_f32(bits);
-                    p0 += vv * lds_q[2 * (c0 + j)];
-                    p1 += vv * lds_q[2 * (c0 + j) + 1];
-                }
-            }
-            if (ZOUT)
-                *(uint2*)(a.zout + pix * 32 + c0) =
-                    make_uint2(f32x2_to_bf16x2(z[0], z[1]), f32x2_to_bf16x2(z[2], z[3]));
-            if (VOUT) *(uint2*)(a.vout + pix * 32 + c0) = vb;
-        }
-        if (PROJ) {
-            p0 += __shfl_xor(p0, 32);
-            p1 += __shfl_xor(p1, 32);
-            if (h == 0) *(float2*)(a.out + pix * 2) = make_float2(p0 + a.qb[0], p1 + a.qb[1]);
-        }
+    };
+    if (r_beg >= r_end) return;
+    bf16x8_t va[4][K1F], vb[4][K1F];
+    load_v(r_beg, va);
+    uint2 ua = load_u(r_beg), ub;
+    for (int64_t row = r_beg; row < r_end; row += 2) {
+        const int64_t r1 = row + 1 < r_end ? row + 1 : row;
+        load_v(r1, vb);
+        ub = load_u(r1);
+        body(row, va, ua);
+        if (row + 1 >= r_end) break;
+        const int64_t r2 = row + 2 < r_end ? row + 2 : row + 1;
+        load_v(r2, va);
+        ua = load_u(r2);
+        body(row + 1, vb, ub);
     }
-    if (!TOUT) continue;
-    __syncthreads();
-    // T^T (c, m2) = sum_y v'^T (c, y) Fy^T (y, m2), each wave over its tiles
-    f32x16_t acc = {};
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int y0 = 32 * (w + 4 * i);
-        if (y0 >= n) break;
-#pragma unroll
-        for (int f = 0; f < 2; ++f) {
-            const bf16x8_t fa = *(const bf16x8_t*)(vT + l32 * vts + y0 + 16 * f + 8 * h);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa, ff[i][f], acc, 0, 0, 0);
-        }
-    }
-    __syncthreads();   // vT is reused for the partial sums of the four waves
-    float* red = (float*)vT;
-#pragma unroll
-    for (int v = 0; v < 16; ++v) {
-        const int c = 8 * (v >> 2) + 4 * h + (v & 3);
-        red[(w * 32 + c) * 33 + l32] = acc[v];
-    }
-    __syncthreads();
-    // T leaves as bf16: its one consumer, the x-transform, rounds it to bf16
-    uint32_t* trow = (uint32_t*)(a.T + row * m2x2 * 32);
-    for (int i = threadIdx.x; i < m2x2 * 16; i += 256) {
-        const int m = i >> 4, c = 2 * (i & 15);
-        float s0 = 0.f, s1 = 0.f;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            s0 += red[(q * 32 + c) * 33 + m];
-            s1 += red[(q * 32 + c + 1) * 33 + m];
-        }
-        trow[i] = f32x2_to_bf16x2(s0, s1);
-    }
-  }
 }
 
 // ---------------------------------------------------------------------------
