@@ -428,6 +428,9 @@ int launch_col_mul_shape(float2* data, int ld, int ncols, int64_t img_stride, in
         }
     }
     const int64_t n_tiles = (int64_t)dc.live_per_img * n_img;
+    if (n_tiles >= ((int64_t)1 << 31))
+        return fail(EMSIM_HIP_EINVAL, "column pass: %lld column tiles, at most 2^31 - 1",
+                    (long long)n_tiles);
     const int64_t tpb = col_tiles_per_block(n_tiles);
     // blocks pair up on an XCD (see k_col_fft_mul_ifft): a multiple of 16
     const dim3 grid((unsigned)(ceil_div(ceil_div(n_tiles, tpb), 16) * 16));

@@ -886,17 +886,24 @@ __global__ __launch_bounds__(RowCfg<N>::BLOCK, EMSIM_PASSA_MIN_WAVES) void k_sli
     float* re = lds_re + gi * Cfg::WORDS;
     float* im = lds_im + gi * Cfg::WORDS;
     const int64_t first = (int64_t)blockIdx.x * rpw * Cfg::G + gi;
+    if (n_pairs <= 0) return;
+    // Loads are never behind a branch: an item past n_pairs (the last
+    // blocks' tail) reads pair n_pairs - 1, the last item's prefetch reads
+    // its own pair again, and ok only guards the stores.  A load behind a
+    // branch would make the compiler wait for every load outstanding -- the
+    // next pair's prefetch included -- before each use of a row's values.
+    auto clamp = [&](int64_t p) { return p < n_pairs ? p : n_pairs - 1; };
     TwCache<N, EPT> twc;
     twc.load(tw, t);
     float2 ya[EPT], yb[EPT], w0[EPT], w1[EPT];
-    op.load_y(first, t, first < n_pairs, ya, yb);
-    if constexpr (!FIRST) op.load_w(first, 0, t, first < n_pairs, w0);
+    op.load_y(clamp(first), t, true, ya, yb);
+    if constexpr (!FIRST) op.load_w(clamp(first), 0, t, true, w0);
     const uint32_t live = op.live_mask(t);
     twc.fence();
     for (int r = 0; r < rpw; ++r) {
         const int64_t it = first + (int64_t)r * Cfg::G;
         const bool ok = it < n_pairs;
-        if constexpr (!FIRST) op.load_w(it, 1, t, ok, w1);
+        if constexpr (!FIRST) op.load_w(clamp(it), 1, t, true, w1);
         float2 z[EPT];
         op.pack(ya, yb, t, z);
         fft_n<N, EPT, true>(z, re, im, t, twc);
@@ -909,11 +916,9 @@ __global__ __launch_bounds__(RowCfg<N>::BLOCK, EMSIM_PASSA_MIN_WAVES) void k_sli
         op.store_v(it, t, ok, va, vb);
         op.transmit_row(w0, va, t, re, im, twc);
         op.store_w(it, 0, t, ok, live, w0);
-        if (r + 1 < rpw) {
-            const int64_t nx = it + Cfg::G;
-            op.load_y(nx, t, nx < n_pairs, ya, yb);
-            if constexpr (!FIRST) op.load_w(nx, 0, t, nx < n_pairs, w0);
-        }
+        const int64_t nx = clamp(r + 1 < rpw ? it + Cfg::G : it);
+        op.load_y(nx, t, true, ya, yb);
+        if constexpr (!FIRST) op.load_w(nx, 0, t, true, w0);
         op.transmit_row(w1, vb, t, re, im, twc);
         op.store_w(it, 1, t, ok, live, w1);
     }

@@ -387,38 +387,42 @@ __global__ __launch_bounds__((ColCfg<N, SHAPE>::BLOCK), (col_min_waves<N, SHAPE>
     mul.prepare(lds_px, threadIdx.x, Cfg::BLOCK);
     if constexpr (Mul::kTable) __syncthreads();
 
-    // ok: the column exists (stored); a dead one is not loaded (its
-    // product with the propagator is zero)
-    auto locate = [&](int64_t g, int& col, bool& ok) -> float2* {
-        const int64_t img = g / dc.live_per_img;
-        col = dc.tile((int)(g - img * dc.live_per_img)) * Cfg::C + c;
-        ok = g < n_tiles && col < ncols;
-        return data + img * img_stride + col;
+    // Tile r of this block.  Its loads are never behind a branch: a tile
+    // index past n_tiles (the last blocks' tail) reads tile n_tiles - 1 and
+    // a column past ncols reads column ncols - 1, and ok (tile and column
+    // exist) only guards the stores.  A load behind a branch would make the
+    // compiler wait for every load outstanding -- the next tile's included --
+    // before the first use of this tile's values.  Tile indices are 32-bit
+    // (the launch keeps n_tiles below 2^31).
+    auto tile_at = [&](int r, int& col, bool& ok) __attribute__((always_inline)) -> float2* {
+        const uint32_t g0 = (uint32_t)col_tile(blockIdx.x, tpb, r);
+        const bool tile_ok = g0 < (uint32_t)n_tiles;
+        const uint32_t g = tile_ok ? g0 : (uint32_t)n_tiles - 1u;
+        const uint32_t img = g / (uint32_t)dc.live_per_img;
+        col = dc.tile((int)(g - img * (uint32_t)dc.live_per_img)) * Cfg::C + c;
+        ok = tile_ok && col < ncols;
+        return data + (int64_t)img * img_stride + (col < ncols ? col : ncols - 1);
     };
     TwCache<N, EPT> twc;
     twc.load(tw, t);
     float2 cur[EPT], nxt[EPT];
     int col;
     bool ok;
-    float2* base = locate(col_tile(blockIdx.x, tpb, 0), col, ok);
-    {
-        const bool live = ok && !dc.dead(col);
+    float2* base = tile_at(0, col, ok);
 #pragma unroll
-        for (int m = 0; m < EPT; ++m)
-            cur[m] = live ? base[(int64_t)(t + m * Cfg::T) * ld] : make_float2(0.f, 0.f);
-    }
+    for (int m = 0; m < EPT; ++m) cur[m] = base[(int64_t)(t + m * Cfg::T) * ld];
     twc.fence();
     for (int r = 0; r < tpb; ++r) {
-        int col_n = 0;
-        bool ok_n = false;
-        float2* base_n = nullptr;
-        if (r + 1 < tpb) {
-            base_n = locate(col_tile(blockIdx.x, tpb, r + 1), col_n, ok_n);
-            const bool live_n = ok_n && !dc.dead(col_n);
+        // the next tile (the last one reloads itself, so no branch)
+        int col_n;
+        bool ok_n;
+        float2* base_n = tile_at(r + 1 < tpb ? r + 1 : r, col_n, ok_n);
 #pragma unroll
-            for (int m = 0; m < EPT; ++m)
-                nxt[m] = live_n ? base_n[(int64_t)(t + m * Cfg::T) * ld] : make_float2(0.f, 0.f);
-        }
+        for (int m = 0; m < EPT; ++m) nxt[m] = base_n[(int64_t)(t + m * Cfg::T) * ld];
+        // a dead column is zero after the propagator whatever it held
+        const bool dead = dc.dead(col);
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) cur[m] = dead ? make_float2(0.f, 0.f) : cur[m];
         const float2 py = mul.column(col);
         if constexpr (DB) fft_n_db<N, EPT, false>(cur, re, im, re2, im2, t, twc);
         else fft_n<N, EPT, false>(cur, re, im, t, twc);
@@ -430,13 +434,11 @@ __global__ __launch_bounds__((ColCfg<N, SHAPE>::BLOCK), (col_min_waves<N, SHAPE>
 #pragma unroll
             for (int m = 0; m < EPT; ++m) base[(int64_t)(t + m * Cfg::T) * ld] = cur[m];
         }
-        if (r + 1 < tpb) {
 #pragma unroll
-            for (int m = 0; m < EPT; ++m) cur[m] = nxt[m];
-            base = base_n;
-            col = col_n;
-            ok = ok_n;
-        }
+        for (int m = 0; m < EPT; ++m) cur[m] = nxt[m];
+        base = base_n;
+        col = col_n;
+        ok = ok_n;
     }
 }
 
