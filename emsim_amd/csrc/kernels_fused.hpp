@@ -527,12 +527,15 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
     // H + e of its molecule's Y rows (ldh >= H + n_elems)
     float2* zpark = Y + (int64_t)b * N * ldh + H;
 
-    // stage this (molecule, slice)'s rows (rows of all elements are adjacent)
-    // and its atoms' y bins.  The bucket range comes straight from global
-    // memory (uniform addresses, so scalar loads; readfirstlane keeps the
-    // values in scalar registers and the branches on them scalar), so every
-    // staging load of a thread -- row table, bucket, twiddles -- is in flight
-    // before its first LDS store, and one barrier ends the staging.
+    // stage this (molecule, slice)'s rows (rows of all elements are adjacent),
+    // its atoms' y bins and the twiddles.  The bucket range comes straight
+    // from global memory (uniform addresses, so scalar loads; readfirstlane
+    // keeps the values in scalar registers and the branches on them scalar).
+    // Every staging load of a thread -- row table, twiddles, the bucket's
+    // first 4 x BLOCK entries -- is issued before its first LDS store, with
+    // indices clamped into range rather than loads behind branches, so they
+    // are in flight together and the staging costs about one round trip
+    // after the bucket range; one barrier ends it.
     const int64_t rbase = (((int64_t)b * n_slices + slice) * n_elems) * N;
     const int n_rows = n_elems * N;
     const bool rows_lds = n_rows < SpecLds<N>::kRows;
@@ -543,7 +546,12 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
     // everything in LDS (the usual case), or the bucket in pieces
     const bool all_lds = rows_lds && tw_lds && kcount <= (uint32_t)SpecLds<N>::kBucket;
     constexpr int RP = (SpecLds<N>::kRows + Cfg::BLOCK - 1) / Cfg::BLOCK;
+    constexpr int QP = (SpecLds<N>::kTwq + Cfg::BLOCK - 1) / Cfg::BLOCK;
+    constexpr int WP = kTwInLds ? (N + Cfg::BLOCK - 1) / Cfg::BLOCK : 1;
+    constexpr int BP = 4;
     uint32_t rv[RP];
+    float2 qv[QP], wv[WP];
+    uint16_t bv[BP];
     if (rows_lds) {
 #pragma unroll
         for (int j = 0; j < RP; ++j) {
@@ -551,7 +559,27 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
             rv[j] = row_start[rbase + (i <= n_rows ? i : n_rows)];
         }
     }
-    if (all_lds) stage_lds<4>(lds_bucket, bucket + kbase, kcount, threadIdx.x, Cfg::BLOCK);
+    if (tw_lds) {
+#pragma unroll
+        for (int j = 0; j < QP; ++j) {
+            const int i = (int)threadIdx.x + j * Cfg::BLOCK;
+            qv[j] = tw[(i < n2 ? i : n2 - 1) * tw_step];
+        }
+    }
+    if constexpr (kTwInLds) {
+#pragma unroll
+        for (int j = 0; j < WP; ++j) {
+            const int i = (int)threadIdx.x + j * Cfg::BLOCK;
+            wv[j] = tw[(i < N ? i : N - 1) * (kTwN / N)];
+        }
+    }
+    if (all_lds && kcount > 0) {
+#pragma unroll
+        for (int j = 0; j < BP; ++j) {
+            const uint32_t i = threadIdx.x + (uint32_t)(j * Cfg::BLOCK);
+            bv[j] = bucket[kbase + (i < kcount ? i : kcount - 1)];
+        }
+    }
     if (rows_lds) {
 #pragma unroll
         for (int j = 0; j < RP; ++j) {
@@ -559,9 +587,32 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
             if (i <= n_rows) lds_rows[i] = rv[j];
         }
     }
-    if (tw_lds)
-        for (int i = threadIdx.x; i < n2; i += Cfg::BLOCK) lds_twq[i] = tw[i * tw_step];
-    if constexpr (kTwInLds) TwLds<N>::fill(lds_tw, tw, threadIdx.x, Cfg::BLOCK);
+    if (tw_lds) {
+#pragma unroll
+        for (int j = 0; j < QP; ++j) {
+            const int i = (int)threadIdx.x + j * Cfg::BLOCK;
+            if (i < n2) lds_twq[i] = qv[j];
+        }
+    }
+    if constexpr (kTwInLds) {
+#pragma unroll
+        for (int j = 0; j < WP; ++j) {
+            const int i = (int)threadIdx.x + j * Cfg::BLOCK;
+            if (i < N) lds_tw[i] = wv[j];
+        }
+    }
+    if (all_lds && kcount > 0) {
+#pragma unroll
+        for (int j = 0; j < BP; ++j) {
+            const uint32_t i = threadIdx.x + (uint32_t)(j * Cfg::BLOCK);
+            if (i < kcount) lds_bucket[i] = bv[j];
+        }
+        // the rest of a long bucket (more than BP x BLOCK atoms)
+        const uint32_t done = (uint32_t)(BP * Cfg::BLOCK);
+        if (kcount > done)
+            stage_lds<BP>(lds_bucket + done, bucket + kbase + done, kcount - done, threadIdx.x,
+                          Cfg::BLOCK);
+    }
     __syncthreads();
 
     float2 acc[EPT];
