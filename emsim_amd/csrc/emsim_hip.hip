@@ -339,13 +339,40 @@ int g_row_items_per_block = 0;   // row groups per row-worker block
 int g_col_tiles_per_block = 0;   // column tiles per column-kernel block
 int g_col_shape = 0;             // column-tile shape (ColCfg SHAPE) of the propagator pass
 
-// Grid of the row worker for n_items rows (or row pairs) of length N.
+// Grid of the row worker for n_items rows (or row pairs) of length N.  With
+// resident > 0 (blocks of this kernel the device holds at once), the fewest
+// rows per block that still fit the grid in one round of resident blocks, so
+// no partial second round trails the first; otherwise rows_per_block.
 template <int N>
-void row_grid(int64_t n_items, int& rpw, dim3& grid) {
+void row_grid(int64_t n_items, int& rpw, dim3& grid, int64_t resident = 0) {
     using Cfg = RowCfg<N>;
     const int64_t groups = ceil_div(n_items, Cfg::G);
-    rpw = g_row_items_per_block > 0 ? g_row_items_per_block : rows_per_block(groups);
+    if (g_row_items_per_block > 0) {
+        rpw = g_row_items_per_block;
+    } else if (resident > 0) {
+        const int64_t r = ceil_div(groups, resident);
+        rpw = (int)(r < 1 ? 1 : (r > 16 ? 16 : r));
+    } else {
+        rpw = rows_per_block(groups);
+    }
     grid = dim3((unsigned)ceil_div(groups, rpw));
+}
+
+// Blocks of kernel k (block size bs, no dynamic LDS) resident on the current
+// device at once, from its occupancy and CU count; 0 if unknown.  Cached per
+// device (a race between threads stores the same value).
+template <class K>
+int64_t resident_blocks(K k, int bs, int (&cache)[64]) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+    if (cache[dev] == 0) {
+        int per_cu = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, bs, 0) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return 0;
+        cache[dev] = per_cu > 0 && cus > 0 ? per_cu * cus : -1;
+    }
+    return cache[dev] > 0 ? cache[dev] : 0;
 }
 
 int64_t col_tiles_per_block(int64_t n_tiles) {
@@ -1427,9 +1454,14 @@ template <int N>
 int launch_slice_pass_a(bool first, const float2* Y, float2* wave, int64_t n_pairs, int ldh,
                         float psi0, float factor, float* vout, int slice, int n_slices, int n1,
                         ColDead cd, const float2* tw, hipStream_t s) {
+    // one round of resident blocks: pass A is latency-bound, and a partial
+    // second round (or a grid below the resident count) leaves SIMDs idle
+    static int resident_cache[64];
+    const int64_t resident =
+        resident_blocks(k_slice_pass_a<N, false>, RowCfg<N>::BLOCK, resident_cache);
     int rpw;
     dim3 grid;
-    row_grid<N>(n_pairs, rpw, grid);
+    row_grid<N>(n_pairs, rpw, grid, resident);
     const dim3 block(RowCfg<N>::BLOCK);
     const int64_t plane = (int64_t)n1 * N;
     if (first)
