@@ -399,9 +399,12 @@ template <int N>
 int launch_row_transmit_fft(bool first, const float2* in, const float* V, float2* out,
                             int64_t rows, int rows_per_img, int64_t v_img_stride, float factor,
                             const float2* tw, hipStream_t s) {
+    static int resident_cache[64];
+    const int64_t resident =
+        resident_blocks(k_row_transmit_fft<N, false>, RowCfg<N>::BLOCK, resident_cache);
     int rpw;
     dim3 grid;
-    row_grid<N>(rows, rpw, grid);
+    row_grid<N>(rows, rpw, grid, resident);
     const dim3 block(RowCfg<N>::BLOCK);
     if (first)
         EMSIM_LAUNCH(N, (k_row_transmit_fft<N, true>), grid, block, 0, s,
@@ -417,9 +420,12 @@ int launch_row_transmit_fft(bool first, const float2* in, const float* V, float2
 template <int N>
 int launch_row_ifft_intensity(const float2* in, float* out, int64_t rows, const float2* tw,
                               hipStream_t s) {
+    static int resident_cache[64];
+    const int64_t resident =
+        resident_blocks(k_row_ifft_intensity<N>, RowCfg<N>::BLOCK, resident_cache);
     int rpw;
     dim3 grid;
-    row_grid<N>(rows, rpw, grid);
+    row_grid<N>(rows, rpw, grid, resident);
     EMSIM_LAUNCH(N, (k_row_ifft_intensity<N>), grid, dim3(RowCfg<N>::BLOCK), 0, s,
                  (OpIfftIntensity<N>{in, out}), rows, rpw, tw);
     return EMSIM_HIP_OK;

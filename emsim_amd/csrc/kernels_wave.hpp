@@ -180,16 +180,22 @@ __device__ __forceinline__ void row_worker(const Op& op, int64_t n_items, int rp
     float* re = lds_re + gi * Cfg::WORDS;
     float* im = lds_im + gi * Cfg::WORDS;
     const int64_t first = (int64_t)blockIdx.x * rpw * Cfg::G + gi;
+    if (n_items <= 0) return;
+    // Loads are never behind a branch: an item past n_items reads item
+    // n_items - 1 and the last item's prefetch reads itself again; ok only
+    // guards the stores.  A load behind a branch would make the compiler
+    // wait for the next item's prefetch before this item's first use.
+    auto clamp = [&](int64_t i) { return i < n_items ? i : n_items - 1; };
     TwCache<N, Cfg::EPT> twc;
     twc.load(tw, t);
     typename Op::Regs cur, nxt;
-    op.load(first, t, first < n_items, cur);
+    op.load(clamp(first), t, true, cur);
     twc.fence();
     for (int r = 0; r < rpw; ++r) {
         const int64_t it = first + (int64_t)r * Cfg::G;
-        if (r + 1 < rpw) op.load(it + Cfg::G, t, it + Cfg::G < n_items, nxt);
+        op.load(clamp(r + 1 < rpw ? it + Cfg::G : it), t, true, nxt);
         op.run(it, t, it < n_items, cur, re, im, twc);
-        if (r + 1 < rpw) cur = nxt;
+        cur = nxt;
     }
 }
 
