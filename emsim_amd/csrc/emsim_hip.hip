@@ -461,6 +461,7 @@ int launch_col_mul_shape(float2* data, int ld, int ncols, int64_t img_stride, in
         }
     }
     const int64_t n_tiles = (int64_t)dc.live_per_img * n_img;
+    if (n_tiles <= 0) return EMSIM_HIP_OK;   // nothing to transform
     if (n_tiles >= ((int64_t)1 << 31))
         return fail(EMSIM_HIP_EINVAL, "column pass: %lld column tiles, at most 2^31 - 1",
                     (long long)n_tiles);

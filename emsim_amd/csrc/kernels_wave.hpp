@@ -390,6 +390,7 @@ __global__ __launch_bounds__((ColCfg<N, SHAPE>::BLOCK), (col_min_waves<N, SHAPE>
     float* im = lds_im + c * Cfg::STRIDE;
     float* re2 = lds_re2 + (DB ? c * Cfg::STRIDE : 0);
     float* im2 = lds_im2 + (DB ? c * Cfg::STRIDE : 0);
+    if (n_tiles <= 0) return;
     mul.prepare(lds_px, threadIdx.x, Cfg::BLOCK);
     if constexpr (Mul::kTable) __syncthreads();
 
