@@ -475,8 +475,8 @@ __device__ __forceinline__ void row_dft_big(float2 (&v)[EPT], bool valid, int r0
 // Column transform of the spectrum kernel, twiddles from LDS or the table.
 template <int N, int EPT, bool INV, bool LDS>
 __device__ __forceinline__ void spec_fft(float2 (&v)[EPT], float* re, float* im, int t,
-                                         const float2* lds_tw, const float2* __restrict__ tw) {
-    if constexpr (LDS) fft_n<N, EPT, INV>(v, re, im, t, TwLds<N>{lds_tw});
+                                         const TwMixed<N, EPT>& twm, const float2* __restrict__ tw) {
+    if constexpr (LDS) fft_n<N, EPT, INV>(v, re, im, t, twm);
     else fft_n<N, EPT, INV>(v, re, im, t, tw);
 }
 
@@ -580,6 +580,11 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
             bv[j] = bucket[kbase + (i < kcount ? i : kcount - 1)];
         }
     }
+    // the column transforms' twiddles: the first twiddled pass's from
+    // registers, the rest from the LDS table
+    TwMixed<N, EPT> twm;
+    twm.p = lds_tw;
+    if constexpr (kTwInLds) twm.load(tw, t);
     if (rows_lds) {
 #pragma unroll
         for (int j = 0; j < RP; ++j) {
@@ -706,7 +711,7 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
                 row_dft_pieces<EPT, Cfg::T, false>(v, valid, e * N + t, t, col, n2 - 1,
                                                    row_start + rbase, bucket, lds_bucket, tw,
                                                    tw_step, Cfg::BLOCK, packed);
-            spec_fft<N, EPT, false, kTwInLds>(v, re, im, t, lds_tw, tw);
+            spec_fft<N, EPT, false, kTwInLds>(v, re, im, t, twm, tw);
         }
 #pragma unroll
         for (int m = 0; m < EPT; ++m) {
@@ -799,7 +804,7 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
         }
         __syncthreads();
     }
-    spec_fft<N, EPT, true, kTwInLds>(acc, re, im, t, lds_tw, tw);
+    spec_fft<N, EPT, true, kTwInLds>(acc, re, im, t, twm, tw);
     if (valid) {
         float2* dst = Y + (int64_t)b * N * ldh + col;
         if (!packed) {
