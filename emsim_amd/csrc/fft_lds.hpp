@@ -172,31 +172,6 @@ __device__ __forceinline__ float2 tw_get(const TwLds<N>& c, int q, int r, int k)
     return c.p[k * r * (N / (Ns * R))];
 }
 
-// LDS twiddles with the first twiddled pass (Ns = lead radix) in registers:
-// that pass's LDS reads have few distinct addresses that share banks, and
-// its 7 twiddles per butterfly cost 14 VGPRs per butterfly; the later passes
-// read the LDS table.
-template <int N, int EPT>
-struct TwMixed {
-    static constexpr int R0 = lead_radix(N);
-    static constexpr int NB = EPT / 8;
-    const float2* p;   // w_N^j in LDS, j < N
-    float2 w[NB][7];
-    __device__ __forceinline__ void load(const float2* __restrict__ tw, int t) {
-#pragma unroll
-        for (int q = 0; q < NB; ++q) {
-            const int k = (t + q * (N / EPT)) & (R0 - 1);
-#pragma unroll
-            for (int r = 1; r < 8; ++r) w[q][r - 1] = tw[k * r * (kTwN / (R0 * 8))];
-        }
-    }
-};
-template <int N, int EPT, int Ns, int R>
-__device__ __forceinline__ float2 tw_get(const TwMixed<N, EPT>& c, int q, int r, int k) {
-    if constexpr (Ns == TwMixed<N, EPT>::R0 && R == 8) return c.w[q][r - 1];
-    else return c.p[k * r * (N / (Ns * R))];
-}
-
 // One Stockham pass: radix R over sub-transforms of length Ns, EPT values
 // per thread.  Before: v[m] = x[t + m N/EPT].  Butterfly j = t + q N/EPT
 // (q < EPT/R) reads x[j + r N/R] (= v[q + r EPT/R]) and writes

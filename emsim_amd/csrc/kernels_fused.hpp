@@ -472,11 +472,12 @@ __device__ __forceinline__ void row_dft_big(float2 (&v)[EPT], bool valid, int r0
     for (int m = 0; m < EPT; ++m) v[m] = make_float2(sr[m], si[m]);
 }
 
-// Column transform of the spectrum kernel, twiddles from LDS or the table.
-template <int N, int EPT, bool INV, bool LDS>
+// Column transform of the spectrum kernel, twiddles from the thread's cache
+// or the table.
+template <int N, int EPT, bool INV, bool CACHED>
 __device__ __forceinline__ void spec_fft(float2 (&v)[EPT], float* re, float* im, int t,
-                                         const TwMixed<N, EPT>& twm, const float2* __restrict__ tw) {
-    if constexpr (LDS) fft_n<N, EPT, INV>(v, re, im, t, twm);
+                                         const TwCache<N, EPT>& twc, const float2* __restrict__ tw) {
+    if constexpr (CACHED) fft_n<N, EPT, INV>(v, re, im, t, twc);
     else fft_n<N, EPT, INV>(v, re, im, t, tw);
 }
 
@@ -498,8 +499,9 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
     // two buffers used by alternate sparse planes, so a plane's fill never
     // overwrites atoms another wave is still reading
     __shared__ uint32_t lds_atoms[2][kSpecDirect > 0 ? kSpecDirect : 1];
-    // the column transforms' twiddles (N <= 512; longer columns read the table,
-    // as the LDS would then no longer hold two blocks per CU)
+    // w_N^j, the x factors of the direct 2-D DFT of sparse planes (N <= 512;
+    // longer columns read the table, as the LDS would then no longer hold two
+    // blocks per CU, and so do their column transforms)
     constexpr bool kTwInLds = N <= 512;
     __shared__ float2 lds_tw[kTwInLds ? N : 1];
     const unsigned item = spec_item(blockIdx.x, gridDim.x);
@@ -580,11 +582,10 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
             bv[j] = bucket[kbase + (i < kcount ? i : kcount - 1)];
         }
     }
-    // the column transforms' twiddles: the first twiddled pass's from
-    // registers, the rest from the LDS table
-    TwMixed<N, EPT> twm;
-    twm.p = lds_tw;
-    if constexpr (kTwInLds) twm.load(tw, t);
+    // the column transforms' twiddles in registers (N <= 512, 28 VGPRs; the
+    // LDS table's reads shared banks)
+    TwCache<N, EPT> twc;
+    if constexpr (kTwInLds) twc.load(tw, t);
     if (rows_lds) {
 #pragma unroll
         for (int j = 0; j < RP; ++j) {
@@ -711,7 +712,7 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
                 row_dft_pieces<EPT, Cfg::T, false>(v, valid, e * N + t, t, col, n2 - 1,
                                                    row_start + rbase, bucket, lds_bucket, tw,
                                                    tw_step, Cfg::BLOCK, packed);
-            spec_fft<N, EPT, false, kTwInLds>(v, re, im, t, twm, tw);
+            spec_fft<N, EPT, false, kTwInLds>(v, re, im, t, twc, tw);
         }
 #pragma unroll
         for (int m = 0; m < EPT; ++m) {
@@ -804,7 +805,7 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
         }
         __syncthreads();
     }
-    spec_fft<N, EPT, true, kTwInLds>(acc, re, im, t, twm, tw);
+    spec_fft<N, EPT, true, kTwInLds>(acc, re, im, t, twc, tw);
     if (valid) {
         float2* dst = Y + (int64_t)b * N * ldh + col;
         if (!packed) {
