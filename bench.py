@@ -153,7 +153,7 @@ def compulsory_bytes(name: str, ctx) -> float:
     return float("nan")
 
 
-def survey_model_bytes_per_image(n: int, nS: int, nE: int) -> float:
+def dense_reference_bytes_per_image(n: int, nS: int, nE: int) -> float:
     """The reference's dense passes per image at 4 B histograms and 8h B half
     spectra: nS (nE (8 + 16h) + 16h + 4 + 36) + 44 bytes per pixel."""
     h = (n // 2 + 1) / n
@@ -756,9 +756,9 @@ def main():
             "measured_bytes_per_step": round(step_measured) if step_measured_ok else None,
             "measured_frac": round(step_measured / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
             if step_measured_ok else None,
-            "survey_model": {
-                "bytes_per_image": round(survey_model_bytes_per_image(cfg["n"], cfg["slices"],
-                                                                      cfg["n_elems"])),
+            "dense_reference_model": {
+                "bytes_per_image": round(dense_reference_bytes_per_image(cfg["n"], cfg["slices"],
+                                                                         cfg["n_elems"])),
                 "note": "the reference's dense passes (histograms, per-element spectra, "
                         "potentials in HBM); the fused path never moves these bytes, so "
                         "this is a model of the replaced algorithm, not a roofline"},
