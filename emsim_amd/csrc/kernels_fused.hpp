@@ -503,6 +503,7 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
     // longer columns read the table, as the LDS would then no longer hold two
     // blocks per CU, and so do their column transforms)
     constexpr bool kTwInLds = N <= 512;
+    constexpr bool kTwCached = N <= 512 || N == 2048;
     __shared__ float2 lds_tw[kTwInLds ? N : 1];
     const unsigned item = spec_item(blockIdx.x, gridDim.x);
     const int tile = (int)(item / (unsigned)batch);
@@ -582,10 +583,12 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
             bv[j] = bucket[kbase + (i < kcount ? i : kcount - 1)];
         }
     }
-    // the column transforms' twiddles in registers (N <= 512, 28 VGPRs; the
-    // LDS table's reads shared banks)
+    // the column transforms' twiddles in registers (28 VGPRs at N = 512, 42
+    // at 2048, where one block per CU leaves room; the LDS table's reads
+    // shared banks, the global table's are L2 round trips); N = 1024 and
+    // 4096 read the table, as their register budgets have no room
     TwCache<N, EPT> twc;
-    if constexpr (kTwInLds) twc.load(tw, t);
+    if constexpr (kTwCached) twc.load(tw, t);
     if (rows_lds) {
 #pragma unroll
         for (int j = 0; j < RP; ++j) {
@@ -712,7 +715,7 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
                 row_dft_pieces<EPT, Cfg::T, false>(v, valid, e * N + t, t, col, n2 - 1,
                                                    row_start + rbase, bucket, lds_bucket, tw,
                                                    tw_step, Cfg::BLOCK, packed);
-            spec_fft<N, EPT, false, kTwInLds>(v, re, im, t, twc, tw);
+            spec_fft<N, EPT, false, kTwCached>(v, re, im, t, twc, tw);
         }
 #pragma unroll
         for (int m = 0; m < EPT; ++m) {
@@ -805,7 +808,7 @@ __global__ __launch_bounds__(SpecCfg<N>::BLOCK, (N >= 512 && N <= 1024) ? EMSIM_
         }
         __syncthreads();
     }
-    spec_fft<N, EPT, true, kTwInLds>(acc, re, im, t, twc, tw);
+    spec_fft<N, EPT, true, kTwCached>(acc, re, im, t, twc, tw);
     if (valid) {
         float2* dst = Y + (int64_t)b * N * ldh + col;
         if (!packed) {
