@@ -204,6 +204,10 @@ struct FnoRowArgs {
     float inv_psi0, sigma;
     int64_t rows;          // rows of the batch (B n)
     int rpb;               // rows per block
+    // mode bit kRowRelift: the lifting (16 input channels -> 32) recomputed
+    // from the simulation's outputs instead of read as activations
+    const uint16_t* w16_lift;   // (16, 32) bf16, [k][c]
+    const float* bias_lift;     // (32)
 };
 
 constexpr int kFnoWStride = 72;    // bf16 per LDS row of W^T / Uhat^T (144 B, 16-B aligned)
@@ -224,7 +228,9 @@ __host__ __device__ constexpr size_t kFnoRowLds(int n) {
 // MODE bits: the sum has the Uhat term, GELU, v' written, z written, the
 // next y-transform computed, the projection computed.
 constexpr int kRowU = 1, kRowAct = 2, kRowV = 4, kRowZ = 8, kRowT = 16, kRowProj = 32,
-              kRowPack = 64;
+              kRowPack = 64, kRowRelift = 128;
+// LDS of a re-lifting row kernel beyond kFnoRowLds: the lifting's W^T
+constexpr size_t kFnoReliftLds = (size_t)32 * kFnoWStride * 2;
 
 // Lifting input channels 16 f + 8 h .. + 7 of pixel (row, y), built from the
 // exit wave and potentials exactly as k_fno_pack writes them.
@@ -287,8 +293,9 @@ __device__ __forceinline__ void fno_packed_frags(const FnoRowArgs& a, int64_t ro
 // 32-channel packed liftings and the training modes (z written) keep the
 // compiler's choice.
 __host__ __device__ constexpr int fno_rows_min_waves(int k1f, int mode) {
-    return (((mode & kRowU) && (mode & kRowT)) || ((mode & kRowPack) && k1f > 1) ||
-            (mode & kRowZ))
+    return (mode & kRowRelift) ? 2
+           : (((mode & kRowU) && (mode & kRowT)) || ((mode & kRowPack) && k1f > 1) ||
+              (mode & kRowZ))
                ? 1
                : 2;
 }
@@ -296,7 +303,11 @@ __host__ __device__ constexpr int fno_rows_min_waves(int k1f, int mode) {
 template <int K1F, int K2F, int MODE>
 __global__ __launch_bounds__(256, fno_rows_min_waves(K1F, MODE)) void k_fno_rows(FnoRowArgs a) {
     constexpr bool HAS_U = MODE & kRowU, ACT = MODE & kRowAct, VOUT = MODE & kRowV,
-                   ZOUT = MODE & kRowZ, TOUT = MODE & kRowT, PROJ = MODE & kRowProj;
+                   ZOUT = MODE & kRowZ, TOUT = MODE & kRowT, PROJ = MODE & kRowProj,
+                   RELIFT = MODE & kRowRelift;
+    static_assert(!RELIFT || K1F == 2, "re-lifting feeds a 32-channel layer");
+    // fragments a row loads: the layer's input, or (RELIFT) the lifting's
+    constexpr int KIN = RELIFT ? 1 : K1F;
     extern __shared__ __align__(16) uint16_t fno_lds[];
     uint16_t* wT = fno_lds;                         // [c][k]
     uint16_t* uT = wT + 32 * kFnoWStride;           // [c][j]
@@ -353,6 +364,20 @@ __global__ __launch_bounds__(256, fno_rows_min_waves(K1F, MODE)) void k_fno_rows
     }
     if (threadIdx.x < 32) lds_bias[threadIdx.x] = a.bias[threadIdx.x];
     if (PROJ && threadIdx.x < 64) lds_q[threadIdx.x] = bf16_to_f32(a.q16[threadIdx.x]);
+    // RELIFT: the lifting's W^T (16 input channels) past the row LDS, and its bias
+    __shared__ float lds_bias_lift[RELIFT ? 32 : 1];
+    uint16_t* wlT = fno_lds + kFnoRowLds(n) / 2;
+    if constexpr (RELIFT) {
+        if ((int)threadIdx.x < 16 * 8) {
+            const uint2 wq = *(const uint2*)(a.w16_lift + 4 * threadIdx.x);
+            const int k = (4 * (int)threadIdx.x) >> 5, c = (4 * (int)threadIdx.x) & 31;
+            wlT[c * kFnoWStride + k] = (uint16_t)wq.x;
+            wlT[(c + 1) * kFnoWStride + k] = (uint16_t)(wq.x >> 16);
+            wlT[(c + 2) * kFnoWStride + k] = (uint16_t)wq.y;
+            wlT[(c + 3) * kFnoWStride + k] = (uint16_t)(wq.y >> 16);
+        }
+        if (threadIdx.x < 32) lds_bias_lift[threadIdx.x] = a.bias_lift[threadIdx.x];
+    }
 
     // The rows are software-pipelined in pairs: the next row's input
     // fragments and Uhat row are loaded while the current row computes.
@@ -366,9 +391,9 @@ __global__ __launch_bounds__(256, fno_rows_min_waves(K1F, MODE)) void k_fno_rows
     const int u_grp = (int)threadIdx.x % kU4;
     // input fragments of one row; a tile beyond n reads the row's first
     // tile and is never used
-    auto load_v = [&](int64_t row, bf16x8_t (&vf)[4][K1F]) __attribute__((always_inline)) {
-        if constexpr (MODE & kRowPack) {
-            fno_packed_frags<K1F>(a, row, w, l32, h, vf);
+    auto load_v = [&](int64_t row, bf16x8_t (&vf)[4][KIN]) __attribute__((always_inline)) {
+        if constexpr (MODE & (kRowPack | kRowRelift)) {
+            fno_packed_frags<KIN>(a, row, w, l32, h, vf);
         } else {
             const uint16_t* vrow = a.vin + row * n * k1;
 #pragma unroll
@@ -385,7 +410,7 @@ __global__ __launch_bounds__(256, fno_rows_min_waves(K1F, MODE)) void k_fno_rows
     auto load_u = [&](int64_t row) __attribute__((always_inline)) {
         return HAS_U ? *(const uint2*)(a.U + row * m2x2 * 32 + 4 * u_grp) : make_uint2(0u, 0u);
     };
-    auto body = [&](int64_t row, const bf16x8_t (&vf)[4][K1F], uint2 u_cur) __attribute__((always_inline)) {
+    auto body = [&](int64_t row, const bf16x8_t (&vf)[4][KIN], uint2 u_cur) __attribute__((always_inline)) {
         // the previous row is done with uT and with vT (its partial sums of T)
         __syncthreads();
         if (u_lane) {
@@ -402,11 +427,50 @@ __global__ __launch_bounds__(256, fno_rows_min_waves(K1F, MODE)) void k_fno_rows
         for (int i = 0; i < 4; ++i) {
             const int y0 = 32 * (w + 4 * i), y = y0 + l32;
             if (y0 >= n) break;
+            // the layer's input fragments: as loaded, or (RELIFT) the lifting
+            // recomputed exactly as the lifting kernel computes its output --
+            // v = bf16(W_lift x + b_lift) -- and moved from the MFMA result
+            // layout (lane: channels 8 g + 4 h + 0..3 of pixel l32) into the
+            // operand layout (lane: channels 16 f + 8 h + 0..7) by swapping
+            // four-channel groups with the lane 32 apart
+            bf16x8_t lv[K1F];
+            if constexpr (RELIFT) {
+                const bf16x8_t la = *(const bf16x8_t*)(wlT + l32 * kFnoWStride + 8 * h);
+                const f32x16_t zl = __builtin_amdgcn_mfma_f32_32x32x16_bf16(la, vf[i][0], f32x16_t{}, 0, 0, 0);
+                uint32_t q[4][2];
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    const int c0 = 8 * g + 4 * h;
+                    float z[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) z[j] = zl[4 * g + j] + lds_bias_lift[c0 + j];
+                    q[g][0] = f32x2_to_bf16x2(z[0], z[1]);
+                    q[g][1] = f32x2_to_bf16x2(z[2], z[3]);
+                }
+#pragma unroll
+                for (int f = 0; f < K1F; ++f) {
+                    const uint32_t k0 = h ? q[2 * f + 1][0] : q[2 * f][0];
+                    const uint32_t k1 = h ? q[2 * f + 1][1] : q[2 * f][1];
+                    const uint32_t s0 = h ? q[2 * f][0] : q[2 * f + 1][0];
+                    const uint32_t s1 = h ? q[2 * f][1] : q[2 * f + 1][1];
+                    const uint32_t r0 = (uint32_t)__shfl_xor((int)s0, 32);
+                    const uint32_t r1 = (uint32_t)__shfl_xor((int)s1, 32);
+                    uint32_t fr[4];
+                    fr[0] = h ? r0 : k0;
+                    fr[1] = h ? r1 : k1;
+                    fr[2] = h ? k0 : r0;
+                    fr[3] = h ? k1 : r1;
+                    lv[f] = __builtin_bit_cast(bf16x8_t, fr);
+                }
+            } else {
+#pragma unroll
+                for (int f = 0; f < K1F; ++f) lv[f] = vf[i][f < KIN ? f : 0];
+            }
             f32x16_t acc = {};
 #pragma unroll
             for (int f = 0; f < K1F; ++f) {
                 const bf16x8_t fa = *(const bf16x8_t*)(wT + l32 * kFnoWStride + 16 * f + 8 * h);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa, vf[i][f], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa, lv[f], acc, 0, 0, 0);
             }
             if (HAS_U) {
 #pragma unroll
@@ -485,7 +549,20 @@ __global__ __launch_bounds__(256, fno_rows_min_waves(K1F, MODE)) void k_fno_rows
         }
     };
     if (r_beg >= r_end) return;
-    bf16x8_t va[4][K1F], vb[4][K1F];
+    if constexpr (RELIFT) {
+        // the raw inputs are converted as they load, so a prefetch of the
+        // next row's would be waited for at once; Uhat stays one row ahead
+        uint2 u_next = load_u(r_beg);
+        for (int64_t row = r_beg; row < r_end; ++row) {
+            bf16x8_t vf[4][KIN];
+            load_v(row, vf);
+            const uint2 u_cur = u_next;
+            u_next = load_u(row + 1 < r_end ? row + 1 : row);
+            body(row, vf, u_cur);
+        }
+        return;
+    }
+    bf16x8_t va[4][KIN], vb[4][KIN];
     load_v(r_beg, va);
     uint2 ua = load_u(r_beg), ub;
     for (int64_t row = r_beg; row < r_end; row += 2) {
