@@ -375,14 +375,6 @@ int64_t resident_blocks(K k, int bs, int (&cache)[64]) {
     return cache[dev] > 0 ? cache[dev] : 0;
 }
 
-int64_t col_tiles_per_block(int64_t n_tiles) {
-    if (g_col_tiles_per_block > 0) return g_col_tiles_per_block;
-    // the fewest tiles per block that keep the grid within 512 blocks (two
-    // resident per CU) so that no second round of blocks trails the first
-    const int64_t tpb = ceil_div(n_tiles, 512);
-    return tpb < 1 ? 1 : (tpb > 8 ? 8 : tpb);
-}
-
 template <int N>
 int launch_row_fft(bool inv, const float2* in, float2* out, int64_t rows, const float2* tw,
                    hipStream_t s) {
@@ -465,11 +457,28 @@ int launch_col_mul_shape(float2* data, int ld, int ncols, int64_t img_stride, in
     if (n_tiles >= ((int64_t)1 << 31))
         return fail(EMSIM_HIP_EINVAL, "column pass: %lld column tiles, at most 2^31 - 1",
                     (long long)n_tiles);
-    const int64_t tpb = col_tiles_per_block(n_tiles);
-    // blocks pair up on an XCD (see k_col_fft_mul_ifft): a multiple of 16
-    const dim3 grid((unsigned)(ceil_div(ceil_div(n_tiles, tpb), 16) * 16));
+    // Blocks pair up on an XCD over strips of two tiles (see
+    // k_col_fft_mul_ifft).  By default one round of resident blocks (two
+    // per CU) with the strips dealt out evenly, or one strip per pair when
+    // there are fewer strips than that; the tiles-per-block knob instead
+    // sets the strips per pair.  Pairs come in eights (a grid of whole
+    // multiples of 16).
+    static int resident_cache[64];
+    const int64_t strips = ceil_div(n_tiles, 2);
+    int64_t pairs;
+    if (g_col_tiles_per_block > 0) {
+        pairs = ceil_div(strips, g_col_tiles_per_block);
+    } else {
+        const int64_t resident =
+            resident_blocks(k_col_fft_mul_ifft<N, Mul, SHAPE>, Cfg::BLOCK, resident_cache);
+        const int64_t max_pairs = resident >= 16 ? resident / 16 * 8 : 256;
+        pairs = strips < max_pairs ? strips : max_pairs;
+    }
+    pairs = ceil_div(pairs, 8) * 8;
+    const ColSched sched{(int)(strips / pairs), (int)(strips % pairs)};
+    const dim3 grid((unsigned)(2 * pairs));
     EMSIM_LAUNCH(N, (k_col_fft_mul_ifft<N, Mul, SHAPE>), grid, dim3(Cfg::BLOCK), 0, s, data, ld,
-                 ncols, dc, img_stride, n_tiles, (int)tpb, mul, tw);
+                 ncols, dc, img_stride, n_tiles, sched, mul, tw);
     return EMSIM_HIP_OK;
 }
 

@@ -325,22 +325,33 @@ inline int rows_per_block(int64_t n_groups) {
 // column kernel: FFT_col -> pointwise Mul(k1, j) -> IFFT_col, in place.
 // n_img images of N rows x ld complex (image stride img_stride); the first
 // ncols columns of each image are transformed in tiles of C columns (tiles
-// numbered image-major).  Each block handles tpb tiles and loads the next
-// tile while transforming the current one.
+// numbered image-major).  Each block handles a run of tiles and loads the
+// next tile while transforming the current one.
 //
 // Workgroups go to the 8 XCDs round-robin by linear id, so blocks L and L + 8
 // share an XCD and start together.  They take the even and the odd tiles of
 // the same strips of 2 C columns (128 bytes of a row when C = 8): block pair
-// p = (L / 16) * 8 + L % 8 owns tiles 2 tpb p .. 2 tpb p + 2 tpb - 1, and
-// (L / 8) % 2 picks the parity.  Both halves of a cache line are then read
-// while the line is in that XCD's L2, instead of the second half being
-// fetched again from HBM a tile later.  The grid is a multiple of 16.
+// p = (L / 16) * 8 + L % 8 owns a run of strips and (L / 8) % 2 picks the
+// parity.  Both halves of a cache line are then read while the line is in
+// that XCD's L2, instead of the second half being fetched again from HBM a
+// tile later.  The grid is a multiple of 16.  The strips are dealt out as
+// evenly as the pair count allows (ColSched: the first `extra` pairs take
+// base + 1 strips, the rest base), so with a grid of two blocks per CU the
+// pairs dispatched first (the larger runs) share their CUs with the pairs
+// dispatched second (the smaller ones) and every CU gets about the same
+// number of tiles.
 // ---------------------------------------------------------------------------
 constexpr unsigned kColXcds = 8;
-__device__ __forceinline__ int64_t col_tile(unsigned L, int tpb, int r) {
+struct ColSched {
+    int base, extra;   // strips per block pair: base + (pair < extra)
+};
+__device__ __forceinline__ void col_run(unsigned L, ColSched sc, int& start, int& count,
+                                        int& parity) {
     const unsigned x = L % kColXcds, k = L / kColXcds;
-    const int64_t pair = (int64_t)(k / 2) * kColXcds + x;
-    return (pair * tpb + r) * 2 + (k % 2);
+    const int pair = (int)((k / 2) * kColXcds + x);
+    count = sc.base + (pair < sc.extra ? 1 : 0);
+    start = pair * sc.base + (pair < sc.extra ? pair : sc.extra);
+    parity = (int)(k % 2);
 }
 #ifndef EMSIM_COL_DB
 #define EMSIM_COL_DB 1
@@ -373,7 +384,7 @@ constexpr int col_min_waves() {
 template <int N, class Mul, int SHAPE = 0>
 __global__ __launch_bounds__((ColCfg<N, SHAPE>::BLOCK), (col_min_waves<N, SHAPE>())) void k_col_fft_mul_ifft(
     float2* data, int ld, int ncols, DeadCols dc, int64_t img_stride, int64_t n_tiles,
-    int tpb, Mul mul, const float2* __restrict__ tw) {
+    ColSched sched, Mul mul, const float2* __restrict__ tw) {
     using Cfg = ColCfg<N, SHAPE>;
     constexpr int EPT = Cfg::EPT;
     // a second plane set where the exchange count allows alternating sets
@@ -394,6 +405,8 @@ __global__ __launch_bounds__((ColCfg<N, SHAPE>::BLOCK), (col_min_waves<N, SHAPE>
     mul.prepare(lds_px, threadIdx.x, Cfg::BLOCK);
     if constexpr (Mul::kTable) __syncthreads();
 
+    int run_start, run_count, parity;
+    col_run(blockIdx.x, sched, run_start, run_count, parity);
     // Tile r of this block.  Its loads are never behind a branch: a tile
     // index past n_tiles (the last blocks' tail) reads tile n_tiles - 1 and
     // a column past ncols reads column ncols - 1, and ok (tile and column
@@ -402,7 +415,7 @@ __global__ __launch_bounds__((ColCfg<N, SHAPE>::BLOCK), (col_min_waves<N, SHAPE>
     // before the first use of this tile's values.  Tile indices are 32-bit
     // (the launch keeps n_tiles below 2^31).
     auto tile_at = [&](int r, int& col, bool& ok) __attribute__((always_inline)) -> float2* {
-        const uint32_t g0 = (uint32_t)col_tile(blockIdx.x, tpb, r);
+        const uint32_t g0 = (uint32_t)(2 * (run_start + r) + parity);
         const bool tile_ok = g0 < (uint32_t)n_tiles;
         const uint32_t g = tile_ok ? g0 : (uint32_t)n_tiles - 1u;
         const uint32_t img = g / (uint32_t)dc.live_per_img;
@@ -419,11 +432,11 @@ __global__ __launch_bounds__((ColCfg<N, SHAPE>::BLOCK), (col_min_waves<N, SHAPE>
 #pragma unroll
     for (int m = 0; m < EPT; ++m) cur[m] = base[(int64_t)(t + m * Cfg::T) * ld];
     twc.fence();
-    for (int r = 0; r < tpb; ++r) {
+    for (int r = 0; r < run_count; ++r) {
         // the next tile (the last one reloads itself, so no branch)
         int col_n;
         bool ok_n;
-        float2* base_n = tile_at(r + 1 < tpb ? r + 1 : r, col_n, ok_n);
+        float2* base_n = tile_at(r + 1 < run_count ? r + 1 : r, col_n, ok_n);
 #pragma unroll
         for (int m = 0; m < EPT; ++m) nxt[m] = base_n[(int64_t)(t + m * Cfg::T) * ld];
         // a dead column is zero after the propagator whatever it held
