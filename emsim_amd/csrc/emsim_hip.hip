@@ -475,7 +475,7 @@ int launch_col_mul_shape(float2* data, int ld, int ncols, int64_t img_stride, in
         pairs = strips < max_pairs ? strips : max_pairs;
     }
     pairs = ceil_div(pairs, 8) * 8;
-    const ColSched sched{(int)(strips / pairs), (int)(strips % pairs)};
+    const RunSched sched{(int)(strips / pairs), (int)(strips % pairs)};
     const dim3 grid((unsigned)(2 * pairs));
     EMSIM_LAUNCH(N, (k_col_fft_mul_ifft<N, Mul, SHAPE>), grid, dim3(Cfg::BLOCK), 0, s, data, ld,
                  ncols, dc, img_stride, n_tiles, sched, mul, tw);
@@ -1472,24 +1472,32 @@ int launch_slice_pass_a(bool first, const float2* Y, float2* wave, int64_t n_pai
                         ColDead cd, const float2* tw, hipStream_t s) {
     // one round of resident blocks: pass A is latency-bound, and a partial
     // second round (or a grid below the resident count) leaves SIMDs idle
+    // The row-pair groups are dealt out evenly over one round of resident
+    // waves (RunSched), so every SIMD ends up with about the same number of
+    // pairs; the rows-per-block knob instead fixes the blocks' count.
     static int resident_cache[64];
     const int64_t resident =
         resident_blocks(k_slice_pass_a<N, false>, RowCfg<N>::BLOCK, resident_cache);
-    int rpw;
-    dim3 grid;
-    row_grid<N>(n_pairs, rpw, grid, resident);
+    const int64_t groups = ceil_div(n_pairs, RowCfg<N>::G);
+    if (groups <= 0) return EMSIM_HIP_OK;
+    int64_t blocks;
+    if (g_row_items_per_block > 0) blocks = ceil_div(groups, g_row_items_per_block);
+    else if (resident > 0) blocks = groups < resident ? groups : resident;
+    else blocks = ceil_div(groups, rows_per_block(groups));
+    const RunSched sched{(int)(groups / blocks), (int)(groups % blocks)};
+    const dim3 grid((unsigned)blocks);
     const dim3 block(RowCfg<N>::BLOCK);
     const int64_t plane = (int64_t)n1 * N;
     if (first)
         EMSIM_LAUNCH(N, (k_slice_pass_a<N, true>), grid, block, 0, s,
                      (OpSlicePassA<N, true>{Y, wave, ldh, psi0, factor, vout, n_slices * plane,
                                             slice * plane, n1, cd.lo, cd.hi}),
-                     n_pairs, rpw, tw);
+                     n_pairs, sched, tw);
     else
         EMSIM_LAUNCH(N, (k_slice_pass_a<N, false>), grid, block, 0, s,
                      (OpSlicePassA<N, false>{Y, wave, ldh, psi0, factor, vout, n_slices * plane,
                                              slice * plane, n1, cd.lo, cd.hi}),
-                     n_pairs, rpw, tw);
+                     n_pairs, sched, tw);
     return EMSIM_HIP_OK;
 }
 

@@ -930,13 +930,14 @@ struct OpSlicePassA {
 #ifndef EMSIM_PASSA_MIN_WAVES
 #define EMSIM_PASSA_MIN_WAVES 1
 #endif
-// A block (one wave for N <= 512) handles rpw groups of G row pairs.  The
+// A block (one wave for N <= 512) handles a run of groups of G row pairs
+// (RunSched: runs of base or base + 1 groups).  The
 // loads are staged so that each is in flight behind transforms: row 1's wave
 // during the C2R and row 0's transforms, and the next pair's half spectra
 // and row 0 during row 1's transforms.
 template <int N, bool FIRST>
 __global__ __launch_bounds__(RowCfg<N>::BLOCK, EMSIM_PASSA_MIN_WAVES) void k_slice_pass_a(
-    OpSlicePassA<N, FIRST> op, int64_t n_pairs, int rpw, const float2* tw) {
+    OpSlicePassA<N, FIRST> op, int64_t n_pairs, RunSched sched, const float2* tw) {
     using Cfg = RowCfg<N>;
     constexpr int EPT = Cfg::EPT;
     __shared__ float lds_re[Cfg::G * Cfg::WORDS];
@@ -945,7 +946,10 @@ __global__ __launch_bounds__(RowCfg<N>::BLOCK, EMSIM_PASSA_MIN_WAVES) void k_sli
     const int t = threadIdx.x % Cfg::T;
     float* re = lds_re + gi * Cfg::WORDS;
     float* im = lds_im + gi * Cfg::WORDS;
-    const int64_t first = (int64_t)blockIdx.x * rpw * Cfg::G + gi;
+    // this block's run of row-pair groups (G pairs side by side)
+    int run_start, run_count;
+    sched.run((int)blockIdx.x, run_start, run_count);
+    const int64_t first = (int64_t)run_start * Cfg::G + gi;
     if (n_pairs <= 0) return;
     // Loads are never behind a branch: an item past n_pairs (the last
     // blocks' tail) reads pair n_pairs - 1, the last item's prefetch reads
@@ -960,7 +964,7 @@ __global__ __launch_bounds__(RowCfg<N>::BLOCK, EMSIM_PASSA_MIN_WAVES) void k_sli
     if constexpr (!FIRST) op.load_w(clamp(first), 0, t, true, w0);
     const uint32_t live = op.live_mask(t);
     twc.fence();
-    for (int r = 0; r < rpw; ++r) {
+    for (int r = 0; r < run_count; ++r) {
         const int64_t it = first + (int64_t)r * Cfg::G;
         const bool ok = it < n_pairs;
         if constexpr (!FIRST) op.load_w(clamp(it), 1, t, true, w1);
@@ -976,7 +980,7 @@ __global__ __launch_bounds__(RowCfg<N>::BLOCK, EMSIM_PASSA_MIN_WAVES) void k_sli
         op.store_v(it, t, ok, va, vb);
         op.transmit_row(w0, va, t, re, im, twc);
         op.store_w(it, 0, t, ok, live, w0);
-        const int64_t nx = clamp(r + 1 < rpw ? it + Cfg::G : it);
+        const int64_t nx = clamp(r + 1 < run_count ? it + Cfg::G : it);
         op.load_y(nx, t, true, ya, yb);
         if constexpr (!FIRST) op.load_w(nx, 0, t, true, w0);
         op.transmit_row(w1, vb, t, re, im, twc);

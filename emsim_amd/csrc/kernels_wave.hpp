@@ -335,22 +335,25 @@ inline int rows_per_block(int64_t n_groups) {
 // parity.  Both halves of a cache line are then read while the line is in
 // that XCD's L2, instead of the second half being fetched again from HBM a
 // tile later.  The grid is a multiple of 16.  The strips are dealt out as
-// evenly as the pair count allows (ColSched: the first `extra` pairs take
-// base + 1 strips, the rest base), so with a grid of two blocks per CU the
-// pairs dispatched first (the larger runs) share their CUs with the pairs
-// dispatched second (the smaller ones) and every CU gets about the same
-// number of tiles.
+// evenly as the pair count allows (RunSched), so with a grid of two blocks
+// per CU the pairs dispatched first (the larger runs) share their CUs with
+// the pairs dispatched second (the smaller ones) and every CU gets about the
+// same number of tiles.
 // ---------------------------------------------------------------------------
 constexpr unsigned kColXcds = 8;
-struct ColSched {
-    int base, extra;   // strips per block pair: base + (pair < extra)
+// Work units dealt out as evenly as possible: worker u takes base + 1 units
+// if u < extra, else base, starting at u base + min(u, extra).
+struct RunSched {
+    int base, extra;
+    __device__ __forceinline__ void run(int u, int& start, int& count) const {
+        count = base + (u < extra ? 1 : 0);
+        start = u * base + (u < extra ? u : extra);
+    }
 };
-__device__ __forceinline__ void col_run(unsigned L, ColSched sc, int& start, int& count,
+__device__ __forceinline__ void col_run(unsigned L, RunSched sc, int& start, int& count,
                                         int& parity) {
     const unsigned x = L % kColXcds, k = L / kColXcds;
-    const int pair = (int)((k / 2) * kColXcds + x);
-    count = sc.base + (pair < sc.extra ? 1 : 0);
-    start = pair * sc.base + (pair < sc.extra ? pair : sc.extra);
+    sc.run((int)((k / 2) * kColXcds + x), start, count);
     parity = (int)(k % 2);
 }
 #ifndef EMSIM_COL_DB
@@ -384,7 +387,7 @@ constexpr int col_min_waves() {
 template <int N, class Mul, int SHAPE = 0>
 __global__ __launch_bounds__((ColCfg<N, SHAPE>::BLOCK), (col_min_waves<N, SHAPE>())) void k_col_fft_mul_ifft(
     float2* data, int ld, int ncols, DeadCols dc, int64_t img_stride, int64_t n_tiles,
-    ColSched sched, Mul mul, const float2* __restrict__ tw) {
+    RunSched sched, Mul mul, const float2* __restrict__ tw) {
     using Cfg = ColCfg<N, SHAPE>;
     constexpr int EPT = Cfg::EPT;
     // a second plane set where the exchange count allows alternating sets
